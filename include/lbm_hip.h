@@ -1,0 +1,205 @@
+/* lbm_hip.h — C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann time step.
+ *
+ * One fused step per launch: SoA pull streaming with halfway bounce-back, BGK collision and
+ * Guo body force, on populations stored shifted (g = f - w). In float and double the result is
+ * bit-identical to the reference's sequential v0 solver.
+ *
+ * The reference has no FFI: its boundary is the duck-typed C++ Solver concept consumed by
+ * run_driver (src/common/driver.hpp:5-16,255-300) and by the test templates. Each entry point
+ * below names the member of SolverV3 (src/v3_cuda_opt/solver_v3.hpp:27-57, solver_v3.cu) or of
+ * the v4 StripEngine (src/v4_mpi_cuda/engine.hpp:36-61) that it replaces. The C++ class
+ * lbm::SolverHip in lbm_hip_solver.hpp restores SolverV3's exact method set on top of it.
+ *
+ * Conventions
+ *   - Every function returns an int status: LBM_OK, or a negative code; the message of the last
+ *     failure on the calling thread is lbm_last_error(). Nothing throws or aborts across the ABI
+ *     (the reference throws std::invalid_argument for bad arguments, solver_v3.cu:169-170, and
+ *     aborts on CUDA errors, cuda_check.hpp:17-25).
+ *   - Host arrays are caller-owned. "real" arrays are float for LBM_FLOAT and LBM_FP16 and
+ *     double for LBM_DOUBLE. Grids are row-major, cell = y * nx + x, N = nx * ny.
+ *   - Device memory is owned by the handle. All work runs on one HIP stream per handle
+ *     (default priority); lbm_step is asynchronous, every read-back synchronises.
+ *   - Handles are not thread-safe; distinct handles are independent (no process-global
+ *     parameters: everything a kernel needs is passed by value as a kernel argument).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_EINVAL (-1) /* invalid argument (reference: std::invalid_argument) */
+#define LBM_EHIP (-2)   /* HIP runtime or launch error (reference: CUDA_CHECK abort) */
+#define LBM_ESTATE (-3) /* call out of order (e.g. set_inflow_outflow after init) */
+#define LBM_ENOMEM (-4) /* device allocation failed */
+
+#define LBM_FLOAT 0  /* fp32 state and arithmetic (reference real = float) */
+#define LBM_DOUBLE 1 /* fp64 state and arithmetic (reference LBM_REAL_DOUBLE) */
+#define LBM_FP16 2   /* fp16 storage of scaled g, fp32 arithmetic (no reference counterpart) */
+
+typedef struct lbm_solver lbm_solver;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lbm_last_error(void);
+
+/* Library version string and the device it runs on (hipDeviceProp name, gcnArchName). */
+int lbm_device_info(char* buf, size_t len);
+
+/* ---- plain solver (replaces SolverV3) ------------------------------------------------------ */
+
+/* SolverV3::SolverV3(nx, ny, tau, force_x, force_y, Mask) — solver_v3.cu:166-186.
+ * solid: N bytes, 1 = solid. tau must be > 0.5. Values are converted to the solver's real. */
+int lbm_create(int nx, int ny, double tau, double force_x, double force_y,
+               const uint8_t* solid, int precision, lbm_solver** out);
+/* SolverV3::~SolverV3 — solver_v3.cu:188-192. NULL is accepted. */
+int lbm_destroy(lbm_solver* s);
+
+/* SolverV3::set_inflow_outflow(u_in) — solver_v3.cu:228-232. Open x: inlet at x = 0 with the
+ * equilibrium of (1, u_in, 0), zero-gradient outlet at x = nx - 1. Call before init. */
+int lbm_set_inflow_outflow(lbm_solver* s, double u_in);
+/* SolverV3::init_equilibrium(rho0, ux0, uy0) — solver_v3.cu:194-226 (pull preimage). */
+int lbm_init_equilibrium(lbm_solver* s, double rho0, double ux0, double uy0);
+
+/* Setters (the reference fixes these at construction, solver_v3.cu:166-167). They take effect
+ * at the next step; populations are kept. set_force_field: 2N host reals, planar (fx plane then
+ * fy plane); NULL returns to the uniform force of lbm_set_force. */
+int lbm_set_mask(lbm_solver* s, const uint8_t* solid);
+int lbm_set_force(lbm_solver* s, double force_x, double force_y);
+int lbm_set_force_field(lbm_solver* s, const void* fxy);
+int lbm_set_tau(lbm_solver* s, double tau);
+
+/* Launch shape: tile of block_x columns (a multiple of 64, <= 1024) by rows_per_thread rows,
+ * one thread per column walking the rows (the reference's LBM_BLOCK "bx x by", solver_v3.cu:
+ * 145-153, is read from the environment at creation with the same syntax). */
+int lbm_set_block(lbm_solver* s, int block_x, int rows_per_thread);
+int lbm_get_block(const lbm_solver* s, int* block_x, int* rows_per_thread);
+
+/* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */
+int lbm_step(lbm_solver* s);
+/* SolverV3::run(steps) — solver_v3.cu:280-283: steps, then synchronise. */
+int lbm_run(lbm_solver* s, int steps);
+/* SolverV3::timed_run(steps) — solver_v3.cu:285-298: seconds between two HIP events recorded
+ * on the solver's stream around the steps. */
+int lbm_timed_run(lbm_solver* s, int steps, double* seconds);
+
+/* SolverV3::macroscopic — solver_v3.cu:322-343: rho, ux, uy (N reals each) from the
+ * post-streaming state, computed on the device with v0's operations; solid cells get 0. */
+int lbm_macroscopic(lbm_solver* s, void* rho, void* ux, void* uy);
+/* SolverV3::total_mass — solver_v3.cu:345-355: fluid cells + sum of g, FP64 on the device
+ * (deterministic tree order, not v0's sequential order). */
+int lbm_total_mass(lbm_solver* s, double* mass);
+/* No reference counterpart: FP64 population momentum sum_c sum_i c_i g_i over fluid cells. */
+int lbm_momentum(lbm_solver* s, double* px, double* py);
+/* SolverV3::populations — solver_v3.cu:300-320: 9N reals, post-streaming, v0 AoS layout
+ * (cell * 9 + i), solid cells 0. */
+int lbm_populations(lbm_solver* s, void* out);
+/* SolverV3::populations_at — solver_v3.cu:234-249: the 9 post-streaming reals of one cell. */
+int lbm_populations_at(lbm_solver* s, int x, int y, void* out9);
+/* Upload an arbitrary post-streaming state (v0 AoS layout); stored as its pull preimage. */
+int lbm_set_populations(lbm_solver* s, const void* in);
+/* Raw stored state: the post-collision pull state R, SoA (plane i at i * N), in the storage
+ * type (for LBM_FP16: the scaled halves as uint16). */
+int lbm_get_state(lbm_solver* s, void* out);
+int lbm_set_state(lbm_solver* s, const void* in);
+
+/* SolverV3::threads — 0 on the GPU. */
+int lbm_threads(const lbm_solver* s);
+/* SolverV3::bench_info — solver_v3.cu:357-370: extra JSON fields (no braces). */
+int lbm_bench_info(const lbm_solver* s, char* buf, size_t len);
+int lbm_dims(const lbm_solver* s, int* nx, int* ny, int* precision);
+
+/* FP16 storage scale (power of two): stored half = g * scale. Default 2^12. */
+int lbm_set_fp16_scale(lbm_solver* s, double scale);
+
+/* Autotuner: times `steps` steps for each candidate (block_x, rows_per_thread) and keeps the
+ * fastest on the handle. The state is saved and restored around the trials. */
+int lbm_autotune(lbm_solver* s, int steps, int* best_block_x, int* best_rows, double* best_mlups);
+
+/* Reference poiseuille_force(H, tau, u_max) in the solver precision (geometry.hpp:24-26). */
+double lbm_poiseuille_force(int precision, int H, double tau, double u_max);
+
+/* ---- strip engine: one GPU's share of the v4 strip decomposition -------------------------- */
+/* Mirrors StripEngine (src/v4_mpi_cuda/engine.hpp:36-61, engine_gpu.cu:27-131). A strip holds
+ * ny_local owned rows plus ghost row 0 (below) and ghost row ny_local + 1 (above); no wrap in y;
+ * SoA with N = nx * (ny_local + 2). Halo: planes {2,5,6} of the top owned row go up, planes
+ * {4,7,8} of the bottom owned row go down; 3 * nx reals per message. */
+typedef struct lbm_strip lbm_strip;
+int lbm_strip_create(int nx, int ny_local, int precision, lbm_strip** out);
+int lbm_strip_destroy(lbm_strip* s);
+/* StripParams (strip_update.hpp:23-30): tau, force, open x with inlet velocity u_in. */
+int lbm_strip_set_params(lbm_strip* s, double tau, double force_x, double force_y, int open_x,
+                         double u_in);
+int lbm_strip_upload_solid(lbm_strip* s, const uint8_t* solid_strip);
+int lbm_strip_upload_state(lbm_strip* s, const void* g);
+int lbm_strip_download_state(lbm_strip* s, void* g);
+/* compute_rows(first, last, stream): local rows inclusive, stream 0 = boundary, 1 = interior. */
+int lbm_strip_compute_rows(lbm_strip* s, int first, int last, int stream);
+/* pack_halo: send_up / send_down (3 * nx reals each) from the new state; waits for stream 0. */
+int lbm_strip_pack_halo(lbm_strip* s, void* send_up, void* send_down);
+/* unpack_halo: recv_from_down -> ghost row 0, recv_from_up -> ghost row ny_local + 1. */
+int lbm_strip_unpack_halo(lbm_strip* s, const void* recv_from_down, const void* recv_from_up);
+/* finish_step: wait for both streams, swap the buffers. */
+int lbm_strip_finish_step(lbm_strip* s);
+int lbm_strip_set_block(lbm_strip* s, int block_x, int rows_per_thread);
+/* Seconds of `steps` rounds of compute_rows(1, ny_local) + ghost-row copy from a device halo
+ * (no host staging), HIP events on stream 0: the one-GPU share at its real shape. */
+int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds);
+
+/* ---- batched persistent kernel over many independent domains ------------------------------- */
+/* No reference counterpart. Each domain is a plain solver of its own shape and parameters,
+ * bit-identical to lbm_create + lbm_run on the same inputs. */
+typedef struct lbm_domain_desc {
+  int nx, ny;
+  double tau, force_x, force_y;
+  const uint8_t* solid; /* nx * ny bytes */
+  double rho0, ux0, uy0; /* initial equilibrium */
+} lbm_domain_desc;
+
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(int n_domains, const lbm_domain_desc* domains, int precision,
+                     lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+/* Steps per captured HIP graph (default 100); the graph is rebuilt on change. */
+int lbm_batch_set_graph_steps(lbm_batch* b, int steps_per_graph);
+/* Tile width (multiple of 64), rows per tile and workgroups of the persistent launch. */
+int lbm_batch_set_shape(lbm_batch* b, int tile_x, int tile_rows, int workgroups);
+int lbm_batch_run(lbm_batch* b, int steps);
+int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds);
+int lbm_batch_populations(lbm_batch* b, int domain, void* out);
+int lbm_batch_total_cells(const lbm_batch* b, long long* cells);
+int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
+                       double* best_mlups);
+
+/* ---- discrete adjoint ---------------------------------------------------------------------- */
+/* No reference counterpart. Float forward of T steps with a per-cell force field and tau,
+ * rest initial state; objective J = mean u_x (with the half-force term) over a probe window,
+ * averaged over the last `avg_steps` states. Reverse sweep by the hand-written adjoint step
+ * with binomial checkpointing (at most `snapshots` stored states). */
+typedef struct lbm_adjoint lbm_adjoint;
+int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int steps,
+                       int probe_x0, int probe_y0, int probe_w, int probe_h, int avg_steps,
+                       int snapshots, int precision, lbm_adjoint** out);
+int lbm_adjoint_destroy(lbm_adjoint* a);
+int lbm_adjoint_set_force_field(lbm_adjoint* a, const void* fxy);
+int lbm_adjoint_set_tau(lbm_adjoint* a, double tau);
+/* Forward only: J. */
+int lbm_adjoint_forward(lbm_adjoint* a, double* J, double* seconds);
+/* J and its gradient: dJ/dF (2N reals, planar) and dJ/dtau. */
+int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_dtau,
+                         double* seconds, long long* forward_steps);
+
+/* ---- HBM copy ceiling (the counterpart of bench/gpu_bandwidth.cu) ---------------------------- */
+/* Best-of-reps copy bandwidth (GB/s, 2 * bytes / time) of a grid-stride copy kernel and of
+ * hipMemcpyDtoD on two arrays of `bytes` each. */
+int lbm_copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,353 @@
+"""MI355X (gfx950) D2Q9 lattice-Boltzmann time step: Python host mirror of the C ABI.
+
+The compute path is liblbm_hip.so (hand-written HIP kernels behind the C ABI declared in
+include/lbm_hip.h); this module only binds it with ctypes and mirrors the reference solver
+interface (SolverV3, src/v3_cuda_opt/solver_v3.hpp:27-57) with numpy arrays:
+
+    s = Solver(nx, ny, tau, force_x, force_y, solid, precision="float")
+    s.init_equilibrium(1.0, 0.0, 0.0); s.run(1000)
+    rho, ux, uy = s.macroscopic(); S = s.populations()
+
+Errors keep the reference's split: invalid arguments raise ValueError (std::invalid_argument
+in the reference), runtime/HIP failures raise RuntimeError. There is no CPU fallback: if the
+library is missing, importing the solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liblbm_hip.so"
+
+LBM_OK, LBM_EINVAL, LBM_EHIP, LBM_ESTATE, LBM_ENOMEM = 0, -1, -2, -3, -4
+FLOAT, DOUBLE, FP16 = 0, 1, 2
+_PRECISION = {"float": FLOAT, "f32": FLOAT, "fp32": FLOAT, "double": DOUBLE, "f64": DOUBLE,
+              "fp64": DOUBLE, "fp16": FP16, "half": FP16}
+
+_lib = None
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def library_path() -> pathlib.Path:
+    return LIB_PATH
+
+
+def lib():
+    """The loaded liblbm_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python build.py "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    i, d, vp, sz = C.c_int, C.c_double, C.c_void_p, C.c_size_t
+    ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    llp = C.POINTER(C.c_longlong)
+    sig = {
+        "lbm_last_error": ([], C.c_char_p),
+        "lbm_device_info": ([C.c_char_p, sz], i),
+        "lbm_create": ([i, i, d, d, d, _u8p, i, C.POINTER(vp)], i),
+        "lbm_destroy": ([vp], i),
+        "lbm_set_inflow_outflow": ([vp, d], i),
+        "lbm_init_equilibrium": ([vp, d, d, d], i),
+        "lbm_set_mask": ([vp, _u8p], i),
+        "lbm_set_force": ([vp, d, d], i),
+        "lbm_set_force_field": ([vp, vp], i),
+        "lbm_set_tau": ([vp, d], i),
+        "lbm_set_block": ([vp, i, i], i),
+        "lbm_get_block": ([vp, ip, ip], i),
+        "lbm_step": ([vp], i),
+        "lbm_run": ([vp, i], i),
+        "lbm_timed_run": ([vp, i, dp], i),
+        "lbm_macroscopic": ([vp, vp, vp, vp], i),
+        "lbm_total_mass": ([vp, dp], i),
+        "lbm_momentum": ([vp, dp, dp], i),
+        "lbm_populations": ([vp, vp], i),
+        "lbm_populations_at": ([vp, i, i, vp], i),
+        "lbm_set_populations": ([vp, vp], i),
+        "lbm_get_state": ([vp, vp], i),
+        "lbm_set_state": ([vp, vp], i),
+        "lbm_threads": ([vp], i),
+        "lbm_bench_info": ([vp, C.c_char_p, sz], i),
+        "lbm_dims": ([vp, ip, ip, ip], i),
+        "lbm_set_fp16_scale": ([vp, d], i),
+        "lbm_autotune": ([vp, i, ip, ip, dp], i),
+        "lbm_poiseuille_force": ([i, i, d, d], d),
+        "lbm_copy_ceiling": ([sz, i, dp, dp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+def last_error() -> str:
+    return lib().lbm_last_error().decode()
+
+
+def check(rc: int) -> None:
+    """Maps a C-ABI status to the reference's exception types."""
+    if rc == LBM_OK:
+        return
+    msg = last_error()
+    if rc == LBM_EINVAL:
+        raise ValueError(msg)
+    if rc == LBM_ENOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def precision_code(precision) -> int:
+    if isinstance(precision, int):
+        return precision
+    try:
+        return _PRECISION[precision]
+    except KeyError:
+        raise ValueError(f"unknown precision {precision!r} (float, double or fp16)") from None
+
+
+def real_dtype(precision) -> type:
+    return np.float64 if precision_code(precision) == DOUBLE else np.float32
+
+
+def storage_dtype(precision) -> type:
+    p = precision_code(precision)
+    return np.float64 if p == DOUBLE else (np.uint16 if p == FP16 else np.float32)
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(512)
+    check(lib().lbm_device_info(buf, len(buf)))
+    return buf.value.decode()
+
+
+def poiseuille_force(H: int, tau: float, u_max: float, precision="float") -> float:
+    """Reference poiseuille_force (src/common/geometry.hpp:24-26) in the solver precision."""
+    return lib().lbm_poiseuille_force(precision_code(precision), int(H), float(tau), float(u_max))
+
+
+def copy_ceiling(nbytes: int = 1 << 32, reps: int = 20) -> tuple[float, float]:
+    """HBM copy ceiling (GB/s): grid-stride copy kernel and hipMemcpyDtoD, best of reps."""
+    a, b = C.c_double(), C.c_double()
+    check(lib().lbm_copy_ceiling(int(nbytes), int(reps), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+# ---- geometry (src/common/geometry.cpp) ------------------------------------------------------
+
+def make_channel(nx: int, ny: int) -> np.ndarray:
+    """Solid rows y = 0 and y = ny - 1, periodic in x (geometry.cpp:5-12); flat uint8 mask."""
+    m = np.zeros((ny, nx), dtype=np.uint8)
+    m[0, :] = 1
+    m[ny - 1, :] = 1
+    return m.reshape(-1)
+
+
+def make_cylinder(nx: int, ny: int, xc: float, yc: float, D: float) -> np.ndarray:
+    """Staircase disc (x - xc)^2 + (y - yc)^2 <= D^2/4, evaluated in double (geometry.cpp:14-23)."""
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    dx, dy = x - xc, y - yc
+    return (dx * dx + dy * dy <= 0.25 * D * D).astype(np.uint8).reshape(-1)
+
+
+def add_disc(mask: np.ndarray, nx: int, ny: int, xc: float, yc: float, r: float) -> np.ndarray:
+    """Adds a staircase disc of radius r to a flat mask (in place) and returns it."""
+    return np.maximum(mask, make_cylinder(nx, ny, xc, yc, 2.0 * r), out=mask)
+
+
+# ---- the plain solver ------------------------------------------------------------------------
+
+class Solver:
+    """SolverV3's interface on the MI355X step (src/v3_cuda_opt/solver_v3.hpp:27-57)."""
+
+    def __init__(self, nx, ny, tau, force_x, force_y, solid, precision="float"):
+        self.precision = precision_code(precision)
+        self.dtype = real_dtype(self.precision)
+        self._nx, self._ny = int(nx), int(ny)
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        if self._solid.size != self._nx * self._ny:
+            raise ValueError("solid mask size does not match the grid")
+        h = C.c_void_p()
+        check(lib().lbm_create(self._nx, self._ny, float(tau), float(force_x), float(force_y),
+                               self._solid, self.precision, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.lbm_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # reference members
+    def nx(self):
+        return self._nx
+
+    def ny(self):
+        return self._ny
+
+    def solid(self):
+        return self._solid
+
+    @property
+    def N(self):
+        return self._nx * self._ny
+
+    def set_inflow_outflow(self, u_in):
+        check(lib().lbm_set_inflow_outflow(self._h, float(u_in)))
+
+    def init_equilibrium(self, rho0, ux0, uy0):
+        check(lib().lbm_init_equilibrium(self._h, float(rho0), float(ux0), float(uy0)))
+
+    def step(self, n: int | None = None):
+        """One asynchronous step (reference step()); step(n) = run(n)."""
+        if n is None:
+            check(lib().lbm_step(self._h))
+        else:
+            self.run(n)
+
+    def run(self, steps):
+        check(lib().lbm_run(self._h, int(steps)))
+
+    def timed_run(self, steps) -> float:
+        sec = C.c_double()
+        check(lib().lbm_timed_run(self._h, int(steps), C.byref(sec)))
+        return sec.value
+
+    def macroscopic(self):
+        r, u, v = (np.empty(self.N, dtype=self.dtype) for _ in range(3))
+        check(lib().lbm_macroscopic(self._h, r.ctypes.data, u.ctypes.data, v.ctypes.data))
+        return r, u, v
+
+    def total_mass(self) -> float:
+        m = C.c_double()
+        check(lib().lbm_total_mass(self._h, C.byref(m)))
+        return m.value
+
+    def momentum(self) -> tuple[float, float]:
+        px, py = C.c_double(), C.c_double()
+        check(lib().lbm_momentum(self._h, C.byref(px), C.byref(py)))
+        return px.value, py.value
+
+    def populations(self) -> np.ndarray:
+        out = np.empty(9 * self.N, dtype=self.dtype)
+        check(lib().lbm_populations(self._h, out.ctypes.data))
+        return out
+
+    def populations_at(self, x, y) -> np.ndarray:
+        out = np.empty(9, dtype=self.dtype)
+        check(lib().lbm_populations_at(self._h, int(x), int(y), out.ctypes.data))
+        return out
+
+    def set_populations(self, S):
+        a = np.ascontiguousarray(S, dtype=self.dtype).reshape(-1)
+        if a.size != 9 * self.N:
+            raise ValueError("populations must hold 9 * nx * ny values")
+        check(lib().lbm_set_populations(self._h, a.ctypes.data))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(9 * self.N, dtype=storage_dtype(self.precision))
+        check(lib().lbm_get_state(self._h, out.ctypes.data))
+        return out
+
+    def set_state(self, R):
+        a = np.ascontiguousarray(R, dtype=storage_dtype(self.precision)).reshape(-1)
+        if a.size != 9 * self.N:
+            raise ValueError("state must hold 9 * nx * ny values")
+        check(lib().lbm_set_state(self._h, a.ctypes.data))
+
+    def threads(self) -> int:
+        return lib().lbm_threads(self._h)
+
+    def bench_info(self) -> str:
+        buf = C.create_string_buffer(1024)
+        check(lib().lbm_bench_info(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    # setters (the reference fixes these at construction)
+    def set_mask(self, solid):
+        m = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        if m.size != self.N:
+            raise ValueError("solid mask size does not match the grid")
+        check(lib().lbm_set_mask(self._h, m))
+        self._solid = m
+
+    def set_force(self, fx, fy):
+        check(lib().lbm_set_force(self._h, float(fx), float(fy)))
+
+    def set_force_field(self, fxy):
+        if fxy is None:
+            check(lib().lbm_set_force_field(self._h, None))
+            return
+        a = np.ascontiguousarray(fxy, dtype=self.dtype).reshape(-1)
+        if a.size != 2 * self.N:
+            raise ValueError("force field must hold 2 * nx * ny values (fx plane, fy plane)")
+        self._force = a
+        check(lib().lbm_set_force_field(self._h, a.ctypes.data))
+
+    def set_tau(self, tau):
+        check(lib().lbm_set_tau(self._h, float(tau)))
+
+    def set_block(self, block_x, rows_per_thread):
+        check(lib().lbm_set_block(self._h, int(block_x), int(rows_per_thread)))
+
+    def block(self) -> tuple[int, int]:
+        bx, r = C.c_int(), C.c_int()
+        check(lib().lbm_get_block(self._h, C.byref(bx), C.byref(r)))
+        return bx.value, r.value
+
+    def set_fp16_scale(self, scale):
+        check(lib().lbm_set_fp16_scale(self._h, float(scale)))
+
+    def autotune(self, steps=20):
+        bx, r, m = C.c_int(), C.c_int(), C.c_double()
+        check(lib().lbm_autotune(self._h, int(steps), C.byref(bx), C.byref(r), C.byref(m)))
+        return bx.value, r.value, m.value
+
+
+# ---- analysis (scripts/strouhal.py:57-81 method) ---------------------------------------------
+
+def strouhal(uy: np.ndarray, dt: float, D: float, U: float) -> tuple[float, float, int]:
+    """Shedding Strouhal number of a transverse-velocity probe signal sampled every dt steps.
+
+    FFT peak of the mean-removed signal with a Hann window, 16x zero padding and parabolic
+    refinement on the log spectrum; and the mean period between upward zero crossings (linear
+    interpolation). Returns (St_fft, St_zero_crossing, periods)."""
+    s = np.asarray(uy, dtype=np.float64)
+    s = s - s.mean()
+    n = s.size
+    spec = np.abs(np.fft.rfft(s * np.hanning(n), 16 * n))
+    freqs = np.fft.rfftfreq(16 * n, d=dt)
+    k = int(np.argmax(spec[1:]) + 1)
+    k = min(max(k, 1), spec.size - 2)
+    a, b, c = np.log(spec[k - 1:k + 2] + 1e-300)
+    den = a - 2 * b + c
+    shift = 0.5 * (a - c) / den if den != 0 else 0.0
+    f_fft = (k + shift) * (freqs[1] - freqs[0])
+    t = np.arange(n) * dt
+    up = np.where((s[:-1] < 0) & (s[1:] >= 0))[0]
+    tc = t[up] - s[up] * (t[up + 1] - t[up]) / (s[up + 1] - s[up])
+    periods = np.diff(tc)
+    f_zero = 1.0 / periods.mean() if periods.size else float("nan")
+    return f_fft * D / U, f_zero * D / U, int(periods.size)

@@ -1,0 +1,266 @@
+// FP16-storage form of the D2Q9 step: populations stored as IEEE binary16 of g * S (S a power of
+// two), all arithmetic in fp32. No reference counterpart: it halves the bytes per cell update
+// (36 B instead of 72 B) of the same stencil as step.hip.
+//
+// Arithmetic (not bit-compatible with v0; checked against the FP64 step):
+//   * Moments and collision in scaled units (g' = S g): S drho = sum g', and the equilibrium
+//     w (S drho + rho S (3 cu + 4.5 cu^2 - 1.5 u^2)) needs no per-population rescaling.
+//   * Contracted to FMA, one reciprocal of rho; the Guo source with a uniform force reduces per
+//     direction to fma(B_i, c_i.u, A_i - C_i u.F) with per-direction constants.
+//   * Stores round stochastically to binary16: 13 random bits (a per-cell, per-step hash) are
+//     added to the fp32 pattern, which is then truncated to the 10-bit half mantissa and
+//     converted exactly. Round-to-nearest would be biased the same way in every cell of a
+//     uniform region and would stall the slow diffusive modes once their per-step change drops
+//     below half an ulp; the unbiased rounding keeps the expected update exact.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace lbmhip {
+namespace {
+
+__device__ __forceinline__ uint32_t hash_cell(uint32_t c, uint32_t seed) {
+  uint32_t h = (c ^ seed) * 0x9E3779B1u;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__device__ __forceinline__ uint32_t xorshift(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+
+// fp32 (scaled units) -> binary16 bits, stochastic rounding with 13 random bits.
+__device__ __forceinline__ unsigned short to_half_sr(float v, uint32_t r13) {
+  const uint32_t b = (__builtin_bit_cast(uint32_t, v) + r13) & ~0x1FFFu;
+  return __builtin_bit_cast(unsigned short, static_cast<_Float16>(__builtin_bit_cast(float, b)));
+}
+
+struct FastCollide {
+  float omega, S, invS, hfx, hfy, fx, fy;
+  float A[kQ], B[kQ], C[kQ];  // Guo: src'_i = B_i cu_i + A_i - C_i u.F (scaled units)
+
+  __device__ __forceinline__ void setup(float omega_, float fp, float S_, float invS_, float fx_,
+                                        float fy_) {
+    omega = omega_;
+    S = S_;
+    invS = invS_;
+    fx = fx_;
+    fy = fy_;
+    hfx = 0.5f * fx_;
+    hfy = 0.5f * fy_;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const float w = weight<float>(i);
+      const float cf = kCx[i] * fx_ + kCy[i] * fy_;
+      A[i] = 3.0f * S * fp * w * cf;
+      B[i] = 9.0f * S * fp * w * cf;
+      C[i] = 3.0f * S * fp * w;
+    }
+  }
+
+  __device__ __forceinline__ void operator()(float (&v)[kQ]) const {
+#pragma clang fp contract(fast)
+    const float m0 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])) + v[8];
+    const float mx = (v[1] + v[5] + v[8]) - (v[3] + v[6] + v[7]);
+    const float my = (v[2] + v[5] + v[6]) - (v[4] + v[7] + v[8]);
+    const float rho = 1.0f + m0 * invS;
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    const float ux = (mx * invS + hfx) * ir;
+    const float uy = (my * invS + hfy) * ir;
+    const float usq15 = 1.5f * (ux * ux + uy * uy);
+    const float rhoS = rho * S;
+    const float uF = ux * fx + uy * fy;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const float cu = kCx[i] * ux + kCy[i] * uy;
+      const float poly = (4.5f * cu + 3.0f) * cu - usq15;
+      const float geq = weight<float>(i) * (m0 + rhoS * poly);
+      const float src = B[i] * cu + (A[i] - C[i] * uF);
+      v[i] = v[i] + omega * (geq - v[i]) + src;
+    }
+  }
+};
+
+template <bool kOpen, bool kField>
+__global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
+  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
+  const int nx = p.nx;
+  const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+  const bool in_x = x < nx;
+  const unsigned xc = in_x ? x : nx - 1;
+  constexpr unsigned B = sizeof(uint16_t);
+  const Cols cols{(xc == 0 ? nx - 1 : xc - 1) * B, xc * B,
+                  (xc + 1 == static_cast<unsigned>(nx) ? 0 : xc + 1) * B};
+  const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;
+  const int y0 = p.row0 + ty * p.rows_per_tile;
+  const int y1 = min(y0 + p.rows_per_tile, p.row0 + p.nrows);
+  Planes g, gn;
+  g.init(p.g, p.N);
+  gn.init(p.gn, p.N);
+  FastCollide col;
+  col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
+
+  for (int y = y0; y < y1; ++y) {
+    int ym = y - 1, yp = y + 1;
+    if (p.wrap_y) {
+      ym = ym < 0 ? p.ny - 1 : ym;
+      yp = yp >= p.ny ? 0 : yp;
+    }
+    const Rows rows{static_cast<unsigned>(ym * nx) * B, static_cast<unsigned>(y * nx) * B,
+                    static_cast<unsigned>(yp * nx) * B};
+    float v[kQ];
+    pull_streamed<uint16_t>(g, rows, cols, v);
+    bool active = in_x;
+    if (p.segflags[static_cast<size_t>(y) * p.nseg + seg]) {
+      const uint32_t link = in_x ? p.links[static_cast<size_t>(y) * nx + xc] : kLinkSolid;
+      active = !(link & kLinkSolid);
+      if (active) apply_links<uint16_t, kOpen>(link, g, rows, cols, p.eq_in, v);
+    }
+    if (active) {
+      const uint32_t cell = static_cast<uint32_t>(y) * static_cast<uint32_t>(nx) + xc;
+      if constexpr (kField) {
+        FastCollide c2 = col;
+        c2.setup(p.omega, p.fp, p.scale, p.inv_scale, p.force[cell], p.force[p.N + cell]);
+        c2(v);
+      } else {
+        col(v);
+      }
+      uint32_t r = hash_cell(cell, p.seed);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) {
+        if (i == 2 || i == 4 || i == 6 || i == 8) r = xorshift(r);
+        const uint32_t r13 = (i & 1) ? (r >> 13) & 0x1FFFu : r & 0x1FFFu;
+        Stored<uint16_t>::store_bits(gn.r[i], cols.c, rows.c, to_half_sr(v[i], r13));
+      }
+    }
+  }
+}
+
+// ---- read-back (values returned unscaled, in float) ------------------------------------------
+
+__device__ __forceinline__ bool gather_h(const GatherParamsH& p, int x, int y,
+                                         float (&v)[kQ]) {
+  if (!gather_cell<uint16_t>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in_s, x, y, v)) return false;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) v[i] *= p.inv_scale;
+  return true;
+}
+
+__global__ void gather_fp16_kernel(const GatherParamsH p, float* __restrict__ out) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= p.N) return;
+  float v[kQ];
+  if (!gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v))
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[c * kQ + i] = v[i];
+}
+
+__global__ void gather_one_fp16_kernel(const GatherParamsH p, int x, int y,
+                                       float* __restrict__ out) {
+  float v[kQ];
+  if (!gather_h(p, x, y, v))
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[i] = v[i];
+}
+
+__global__ void macro_fp16_kernel(const GatherParamsH p, float fx, float fy,
+                                  const float* __restrict__ force, float* __restrict__ rho,
+                                  float* __restrict__ ux, float* __restrict__ uy) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= p.N) return;
+  float v[kQ];
+  float r = 0.0f, u = 0.0f, w = 0.0f;
+  if (gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v)) {
+    if (force) {
+      fx = force[c];
+      fy = force[p.N + c];
+    }
+    moments(v, fx, fy, r, u, w);
+  }
+  rho[c] = r;
+  ux[c] = u;
+  uy[c] = w;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+    reduce_fp16_kernel(const GatherParamsH p, double* __restrict__ partials) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < p.N;
+       c += static_cast<size_t>(gridDim.x) * kReduceThreads) {
+    float v[kQ];
+    if (!gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v)) continue;
+    s[0] += 1.0;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const double d = static_cast<double>(v[i]);
+      s[1] += d;
+      s[2] += kCx[i] * d;
+      s[3] += kCy[i] * d;
+    }
+  }
+  block_sum4(s);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = s[k];
+}
+
+}  // namespace
+
+hipError_t launch_step_fp16(const StepParamsH& p, bool open_x, int block_x, hipStream_t stream) {
+  const int tiles_y = (p.nrows + p.rows_per_tile - 1) / p.rows_per_tile;
+  const unsigned grid = static_cast<unsigned>(p.tiles_x) * static_cast<unsigned>(tiles_y);
+  if (grid == 0) return hipSuccess;
+  const bool field = p.force != nullptr;
+  if (open_x) {
+    if (field)
+      step_fp16_kernel<true, true><<<grid, block_x, 0, stream>>>(p);
+    else
+      step_fp16_kernel<true, false><<<grid, block_x, 0, stream>>>(p);
+  } else {
+    if (field)
+      step_fp16_kernel<false, true><<<grid, block_x, 0, stream>>>(p);
+    else
+      step_fp16_kernel<false, false><<<grid, block_x, 0, stream>>>(p);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_fp16(const GatherParamsH& p, float* out, hipStream_t stream) {
+  gather_fp16_kernel<<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_one_fp16(const GatherParamsH& p, int x, int y, float* out9,
+                                  hipStream_t stream) {
+  gather_one_fp16_kernel<<<1, 1, 0, stream>>>(p, x, y, out9);
+  return hipGetLastError();
+}
+
+hipError_t launch_macro_fp16(const GatherParamsH& p, float fx, float fy,
+                             const float* force, float* rho, float* ux, float* uy,
+                             hipStream_t stream) {
+  macro_fp16_kernel<<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, fx, fy, force,
+                                                                                  rho, ux, uy);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_fp16(const GatherParamsH& p, double* partials, double* out,
+                              hipStream_t stream) {
+  reduce_fp16_kernel<<<kReduceBlocks, kReduceThreads, 0, stream>>>(p, partials);
+  reduce_final_kernel<<<1, kReduceThreads, 0, stream>>>(partials, kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbmhip

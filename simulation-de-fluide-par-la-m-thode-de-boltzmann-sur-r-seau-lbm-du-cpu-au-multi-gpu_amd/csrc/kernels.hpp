@@ -1,0 +1,115 @@
+// Kernel parameter blocks and host-side launchers of the D2Q9 step and its read-back kernels.
+//
+// Data layout in HBM (one domain):
+//   g, gn     9 planes of N = nx * ny values (SoA), plane i at i * N, row-major cells
+//             y * nx + x; g holds the post-collision pull state R (src/v3_cuda_opt/
+//             solver_v3.hpp:23-26); the step reads g and writes gn.
+//   links     N uint16 link codes (d2q9.hpp), read only in dirty segments.
+//   segflags  one uint8 per (row, 64-column segment): 1 if any cell of the segment is solid,
+//             pulls from a solid cell, or lies on an open-x edge. Clean segments (the bulk of
+//             a channel) never read the link codes.
+//   force     optional per-cell body force, 2 planes of N (fx, fy).
+// Everything a launch needs is passed by value in these structs (kernel arguments), so two
+// solvers never share parameters.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lbmhip {
+
+template <class T>
+struct StepParams {
+  const T* g;
+  T* gn;
+  const uint16_t* links;
+  const uint8_t* segflags;
+  const T* force;      // nullptr: uniform (fx, fy)
+  size_t N;            // plane stride
+  int nx, ny;          // grid of the buffers (a strip includes its two ghost rows)
+  int row0, nrows;     // rows updated: [row0, row0 + nrows)
+  int rows_per_tile;   // rows walked by one thread
+  int tiles_x;         // ceil(nx / blockDim.x)
+  int nseg;            // ceil(nx / 64)
+  int wrap_y;          // 1: periodic in y; 0: strip (rows row0-1 and row0+nrows exist)
+  T omega, fp, fx, fy; // 1/tau, 1 - omega/2 (host-computed in T, solver_v3.cu:252-253)
+  T eq_in[9];          // inlet equilibrium (open x)
+};
+
+// FP16-storage step: halves hold g * scale; arithmetic in fp32 (fp16.hip).
+struct StepParamsH {
+  const uint16_t* g;
+  uint16_t* gn;
+  const uint16_t* links;
+  const uint8_t* segflags;
+  const float* force;
+  size_t N;
+  int nx, ny;
+  int row0, nrows;
+  int rows_per_tile;
+  int tiles_x;
+  int nseg;
+  int wrap_y;
+  float omega, fp, fx, fy;
+  float scale, inv_scale;
+  float eq_in[9];  // inlet equilibrium, already scaled
+  uint32_t seed;   // stochastic-rounding stream of this step
+};
+
+// Read-back parameters (gather of the post-streaming state), float / double storage.
+template <class T>
+struct GatherParams {
+  const T* g;
+  const uint16_t* links;
+  const T* force;
+  size_t N;
+  int nx, ny;
+  T fx, fy;
+  T eq_in[9];
+};
+
+// Read-back parameters, FP16 storage (values returned unscaled, in float).
+struct GatherParamsH {
+  const uint16_t* g;
+  const uint16_t* links;
+  size_t N;
+  int nx, ny;
+  float eq_in_s[9];  // scaled inlet equilibrium
+  float inv_scale;
+};
+
+// Launchers (step.hip, fp16.hip). All are asynchronous on `stream`.
+template <class T>
+hipError_t launch_step(const StepParams<T>& p, bool open_x, int block_x, hipStream_t stream);
+hipError_t launch_step_fp16(const StepParamsH& p, bool open_x, int block_x, hipStream_t stream);
+
+// out: 9N values, v0 AoS layout (cell * 9 + i), solid cells 0.
+template <class T>
+hipError_t launch_gather(const GatherParams<T>& p, T* out, hipStream_t stream);
+hipError_t launch_gather_fp16(const GatherParamsH& p, float* out, hipStream_t stream);
+template <class T>
+hipError_t launch_gather_one(const GatherParams<T>& p, int x, int y, T* out9, hipStream_t stream);
+hipError_t launch_gather_one_fp16(const GatherParamsH& p, int x, int y, float* out9,
+                                  hipStream_t stream);
+// rho, ux, uy (N each), solid cells 0.
+template <class T>
+hipError_t launch_macro(const GatherParams<T>& p, T* rho, T* ux, T* uy, hipStream_t stream);
+hipError_t launch_macro_fp16(const GatherParamsH& p, float fx, float fy, const float* force,
+                             float* rho, float* ux, float* uy, hipStream_t stream);
+// FP64 sums over fluid cells of the post-streaming state: out[0] = number of fluid cells,
+// out[1] = sum g, out[2] = sum cx g, out[3] = sum cy g. `partials` holds 4 doubles per block
+// of the first pass (kReduceBlocks blocks).
+constexpr int kReduceBlocks = 1024;
+template <class T>
+hipError_t launch_reduce(const GatherParams<T>& p, double* partials, double* out,
+                         hipStream_t stream);
+hipError_t launch_reduce_fp16(const GatherParamsH& p, double* partials, double* out,
+                              hipStream_t stream);
+
+// Grid-stride copy (HBM ceiling), n floats.
+hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block,
+                       hipStream_t stream);
+
+}  // namespace lbmhip

@@ -1,0 +1,468 @@
+// Plain solver operations: allocation, host-side preparation (link codes, pull preimage),
+// launches and read-backs. float and double share one template; FP16 storage has its own
+// launchers (fp16.hip) behind the same state.
+#include "solver_impl.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "d2q9.hpp"
+#include "host_common.hpp"
+#include "kernels.hpp"
+#include "lbm_hip.h"
+
+namespace lbmhip {
+
+namespace {
+
+template <class T>
+StepParams<T> make_step_params(const SolverState& s) {
+  StepParams<T> p{};
+  p.g = static_cast<const T*>(s.g);
+  p.gn = static_cast<T*>(s.gn);
+  p.links = s.links;
+  p.segflags = s.segflags;
+  p.force = static_cast<const T*>(s.force);
+  p.N = s.cells();
+  p.nx = s.nx;
+  p.ny = s.ny;
+  p.row0 = 0;
+  p.nrows = s.ny;
+  p.rows_per_tile = s.rows;
+  p.tiles_x = (s.nx + s.block_x - 1) / s.block_x;
+  p.nseg = s.nseg;
+  p.wrap_y = 1;
+  // omega and the force prefactor computed on the host in the solver real, as the reference
+  // does once per step (src/v3_cuda_opt/solver_v3.cu:252-253).
+  const T tau = static_cast<T>(s.tau);
+  p.omega = T(1) / tau;
+  p.fp = T(1) - T(0.5) * p.omega;
+  p.fx = static_cast<T>(s.fx);
+  p.fy = static_cast<T>(s.fy);
+  T eq[kQ] = {};
+  if (s.open_x) equilibrium_state<T>(T(1), static_cast<T>(s.u_in), T(0), eq);
+  for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i];
+  return p;
+}
+
+template <class T>
+GatherParams<T> make_gather_params(const SolverState& s) {
+  GatherParams<T> p{};
+  p.g = static_cast<const T*>(s.g);
+  p.links = s.links;
+  p.force = static_cast<const T*>(s.force);
+  p.N = s.cells();
+  p.nx = s.nx;
+  p.ny = s.ny;
+  p.fx = static_cast<T>(s.fx);
+  p.fy = static_cast<T>(s.fy);
+  T eq[kQ] = {};
+  if (s.open_x) equilibrium_state<T>(T(1), static_cast<T>(s.u_in), T(0), eq);
+  for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i];
+  return p;
+}
+
+GatherParamsH make_gather_params_h(const SolverState& s) {
+  GatherParamsH p{};
+  p.g = static_cast<const uint16_t*>(s.g);
+  p.links = s.links;
+  p.N = s.cells();
+  p.nx = s.nx;
+  p.ny = s.ny;
+  float eq[kQ] = {};
+  if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
+  for (int i = 0; i < kQ; ++i) p.eq_in_s[i] = eq[i] * static_cast<float>(s.fp16_scale);
+  p.inv_scale = static_cast<float>(1.0 / s.fp16_scale);
+  return p;
+}
+
+StepParamsH make_step_params_h(const SolverState& s) {
+  StepParamsH p{};
+  p.g = static_cast<const uint16_t*>(s.g);
+  p.gn = static_cast<uint16_t*>(s.gn);
+  p.links = s.links;
+  p.segflags = s.segflags;
+  p.force = static_cast<const float*>(s.force);
+  p.N = s.cells();
+  p.nx = s.nx;
+  p.ny = s.ny;
+  p.row0 = 0;
+  p.nrows = s.ny;
+  p.rows_per_tile = s.rows;
+  p.tiles_x = (s.nx + s.block_x - 1) / s.block_x;
+  p.nseg = s.nseg;
+  p.wrap_y = 1;
+  const float tau = static_cast<float>(s.tau);
+  p.omega = 1.0f / tau;
+  p.fp = 1.0f - 0.5f * p.omega;
+  p.fx = static_cast<float>(s.fx);
+  p.fy = static_cast<float>(s.fy);
+  p.scale = static_cast<float>(s.fp16_scale);
+  p.inv_scale = static_cast<float>(1.0 / s.fp16_scale);
+  float eq[kQ] = {};
+  if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
+  for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i] * p.scale;
+  p.seed = s.step_count * 0x9E3779B9u + 0x7F4A7C15u;
+  return p;
+}
+
+// float -> scaled half, round to nearest even (host side, for uploads).
+uint16_t to_half(float f) {
+  _Float16 h = static_cast<_Float16>(f);
+  uint16_t u;
+  std::memcpy(&u, &h, 2);
+  return u;
+}
+
+}  // namespace
+
+size_t SolverState::state_bytes() const { return cells() * kQ * storage_size(prec); }
+
+int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, const uint8_t* solid_,
+                        int prec_) {
+  nx = nx_;
+  ny = ny_;
+  prec = prec_;
+  fx = fx_;
+  fy = fy_;
+  LBM_TRY(set_tau(tau_));
+  if (storage_size(prec) * cells() >= (size_t(1) << 32))
+    return fail(LBM_EINVAL, "one population plane must stay below 4 GiB");
+  solid.assign(solid_, solid_ + cells());
+  LBM_TRY(parse_block_env(&block_x, &rows));
+  LBM_HIP_TRY(hipStreamCreate(&stream));
+  LBM_HIP_TRY(hipEventCreate(&ev0));
+  LBM_HIP_TRY(hipEventCreate(&ev1));
+  const size_t bytes = state_bytes();
+  if (hipMalloc(&g, bytes) != hipSuccess || hipMalloc(&gn, bytes) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the population buffers failed");
+  LBM_HIP_TRY(hipMemset(g, 0, bytes));
+  LBM_HIP_TRY(hipMemset(gn, 0, bytes));
+  if (hipMalloc(&links, cells() * sizeof(uint16_t)) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the link codes failed");
+  nseg = (nx + 63) / 64;
+  if (hipMalloc(&segflags, static_cast<size_t>(ny) * nseg) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the segment flags failed");
+  if (hipMalloc(&red, sizeof(double) * (4 * kReduceBlocks + 4)) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the reduction buffer failed");
+  return upload_links();
+}
+
+void SolverState::release() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (void* p : {g, gn, static_cast<void*>(links), static_cast<void*>(segflags), force, staging,
+                  static_cast<void*>(red)})
+    if (p) (void)hipFree(p);
+  g = gn = force = staging = nullptr;
+  links = nullptr;
+  segflags = nullptr;
+  red = nullptr;
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+  ev0 = ev1 = nullptr;
+  stream = nullptr;
+}
+
+int SolverState::upload_links() {
+  if (open_x && nx < 3) return fail(LBM_EINVAL, "an open domain needs nx >= 3");
+  const Links L = build_links(nx, ny, solid.data(), open_x, true);
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(links, L.code.data(), L.code.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  LBM_HIP_TRY(hipMemcpy(segflags, L.seg.data(), L.seg.size(), hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+int SolverState::set_inflow_outflow(double u) {
+  open_x = true;
+  u_in = u;
+  return upload_links();
+}
+
+int SolverState::set_tau(double t) {
+  const bool ok = prec == LBM_DOUBLE ? t > 0.5 : static_cast<float>(t) > 0.5f;
+  if (!ok || !std::isfinite(t)) return fail(LBM_EINVAL, "tau must be > 0.5");
+  tau = t;
+  return LBM_OK;
+}
+
+int SolverState::set_fp16_scale(double scale) {
+  if (!(scale > 0.0) || std::ldexp(1.0, std::ilogb(scale)) != scale || scale > 65536.0 * 16)
+    return fail(LBM_EINVAL, "the FP16 scale must be a power of two");
+  fp16_scale = scale;
+  return LBM_OK;
+}
+
+int SolverState::set_mask(const uint8_t* s) {
+  if (!s) return fail(LBM_EINVAL, "solid mask size does not match the grid");
+  solid.assign(s, s + cells());
+  return upload_links();
+}
+
+int SolverState::set_force_field(const void* fxy) {
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  if (!fxy) {
+    if (force) LBM_HIP_TRY(hipFree(force));
+    force = nullptr;
+    return LBM_OK;
+  }
+  const size_t bytes = 2 * cells() * real_size(prec);
+  if (!force && hipMalloc(&force, bytes) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the force field failed");
+  LBM_HIP_TRY(hipMemcpy(force, fxy, bytes, hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+int SolverState::init_equilibrium(double rho0, double ux0, double uy0) {
+  const size_t n = cells() * kQ;
+  auto upload = [&](const void* host) -> int {
+    LBM_HIP_TRY(hipStreamSynchronize(stream));
+    LBM_HIP_TRY(hipMemcpy(g, host, state_bytes(), hipMemcpyHostToDevice));
+    LBM_HIP_TRY(hipMemcpy(gn, host, state_bytes(), hipMemcpyHostToDevice));
+    return LBM_OK;
+  };
+  if (prec == LBM_DOUBLE) {
+    double eq[kQ];
+    equilibrium_state<double>(rho0, ux0, uy0, eq);
+    std::vector<double> R(n);
+    preimage_uniform(nx, ny, solid.data(), open_x, eq, R.data());
+    return upload(R.data());
+  }
+  float eq[kQ];
+  equilibrium_state<float>(static_cast<float>(rho0), static_cast<float>(ux0), static_cast<float>(uy0),
+                           eq);
+  std::vector<float> R(n);
+  preimage_uniform(nx, ny, solid.data(), open_x, eq, R.data());
+  if (prec == LBM_FLOAT) return upload(R.data());
+  std::vector<uint16_t> H(n);
+  const float sc = static_cast<float>(fp16_scale);
+  for (size_t k = 0; k < n; ++k) H[k] = to_half(R[k] * sc);
+  return upload(H.data());
+}
+
+int SolverState::set_populations(const void* in) {
+  if (!in) return fail(LBM_EINVAL, "null input array");
+  const size_t n = cells() * kQ;
+  auto upload = [&](const void* host) -> int {
+    LBM_HIP_TRY(hipStreamSynchronize(stream));
+    LBM_HIP_TRY(hipMemcpy(g, host, state_bytes(), hipMemcpyHostToDevice));
+    LBM_HIP_TRY(hipMemcpy(gn, host, state_bytes(), hipMemcpyHostToDevice));
+    return LBM_OK;
+  };
+  if (prec == LBM_DOUBLE) {
+    std::vector<double> R(n);
+    preimage(nx, ny, solid.data(), open_x, static_cast<const double*>(in), R.data());
+    return upload(R.data());
+  }
+  std::vector<float> R(n);
+  preimage(nx, ny, solid.data(), open_x, static_cast<const float*>(in), R.data());
+  if (prec == LBM_FLOAT) return upload(R.data());
+  std::vector<uint16_t> H(n);
+  const float sc = static_cast<float>(fp16_scale);
+  for (size_t k = 0; k < n; ++k) H[k] = to_half(R[k] * sc);
+  return upload(H.data());
+}
+
+int SolverState::step() {
+  hipError_t e;
+  if (prec == LBM_FLOAT)
+    e = launch_step(make_step_params<float>(*this), open_x, block_x, stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_step(make_step_params<double>(*this), open_x, block_x, stream);
+  else
+    e = launch_step_fp16(make_step_params_h(*this), open_x, block_x, stream);
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("step launch: ") + hipGetErrorString(e));
+  std::swap(g, gn);
+  ++step_count;
+  return LBM_OK;
+}
+
+int SolverState::ensure_staging(size_t bytes) {
+  if (staging_bytes >= bytes) return LBM_OK;
+  if (staging) LBM_HIP_TRY(hipFree(staging));
+  staging = nullptr;
+  staging_bytes = 0;
+  if (hipMalloc(&staging, bytes) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the read-back buffer failed");
+  staging_bytes = bytes;
+  return LBM_OK;
+}
+
+int SolverState::populations(void* out) {
+  if (!out) return fail(LBM_EINVAL, "null output array");
+  const size_t bytes = cells() * kQ * real_size(prec);
+  LBM_TRY(ensure_staging(bytes));
+  hipError_t e;
+  if (prec == LBM_FLOAT)
+    e = launch_gather(make_gather_params<float>(*this), static_cast<float*>(staging), stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_gather(make_gather_params<double>(*this), static_cast<double*>(staging), stream);
+  else
+    e = launch_gather_fp16(make_gather_params_h(*this), static_cast<float*>(staging), stream);
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("gather launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(out, staging, bytes, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int SolverState::populations_at(int x, int y, void* out9) {
+  if (!out9) return fail(LBM_EINVAL, "null output array");
+  const size_t bytes = kQ * real_size(prec);
+  LBM_TRY(ensure_staging(bytes));
+  hipError_t e;
+  if (prec == LBM_FLOAT)
+    e = launch_gather_one(make_gather_params<float>(*this), x, y, static_cast<float*>(staging), stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_gather_one(make_gather_params<double>(*this), x, y, static_cast<double*>(staging),
+                          stream);
+  else
+    e = launch_gather_one_fp16(make_gather_params_h(*this), x, y, static_cast<float*>(staging),
+                               stream);
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("gather launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(out9, staging, bytes, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int SolverState::macroscopic(void* rho, void* ux, void* uy) {
+  if (!rho || !ux || !uy) return fail(LBM_EINVAL, "null output array");
+  const size_t rs = real_size(prec), n = cells();
+  LBM_TRY(ensure_staging(3 * n * rs));
+  char* st = static_cast<char*>(staging);
+  hipError_t e;
+  if (prec == LBM_FLOAT)
+    e = launch_macro(make_gather_params<float>(*this), reinterpret_cast<float*>(st),
+                     reinterpret_cast<float*>(st + n * rs), reinterpret_cast<float*>(st + 2 * n * rs),
+                     stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_macro(make_gather_params<double>(*this), reinterpret_cast<double*>(st),
+                     reinterpret_cast<double*>(st + n * rs),
+                     reinterpret_cast<double*>(st + 2 * n * rs), stream);
+  else {
+    GatherParamsH p = make_gather_params_h(*this);
+    e = launch_macro_fp16(p, static_cast<float>(fx), static_cast<float>(fy),
+                          static_cast<const float*>(force), reinterpret_cast<float*>(st),
+                          reinterpret_cast<float*>(st + n * rs),
+                          reinterpret_cast<float*>(st + 2 * n * rs), stream);
+  }
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("macro launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(rho, st, n * rs, hipMemcpyDeviceToHost));
+  LBM_HIP_TRY(hipMemcpy(ux, st + n * rs, n * rs, hipMemcpyDeviceToHost));
+  LBM_HIP_TRY(hipMemcpy(uy, st + 2 * n * rs, n * rs, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int SolverState::reduce(double (&out)[4]) {
+  hipError_t e;
+  double* partials = red;
+  double* result = red + 4 * kReduceBlocks;
+  if (prec == LBM_FLOAT)
+    e = launch_reduce(make_gather_params<float>(*this), partials, result, stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_reduce(make_gather_params<double>(*this), partials, result, stream);
+  else
+    e = launch_reduce_fp16(make_gather_params_h(*this), partials, result, stream);
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("reduce launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(out, result, sizeof(out), hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_mlups) {
+  // Trials run on a copy of the state, which is restored afterwards. Every launch shape
+  // computes the same bits (each cell's arithmetic does not depend on the shape).
+  const size_t bytes = state_bytes();
+  void* backup = nullptr;
+  if (hipMalloc(&backup, bytes) != hipSuccess)
+    return fail(LBM_ENOMEM, "device allocation of the autotune backup failed");
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(backup, g, bytes, hipMemcpyDeviceToDevice));
+  const int keep_bx = block_x, keep_rows = rows;
+  double best = -1.0;
+  int bbx = keep_bx, brows = keep_rows;
+  for (int bx : {64, 128, 256, 512, 1024})
+    for (int r : {1, 2, 4, 8, 16, 32}) {
+      block_x = bx;
+      rows = r;
+      int rc = step();  // warm-up (also checks the launch)
+      if (rc != LBM_OK) continue;
+      LBM_HIP_TRY(hipEventRecord(ev0, stream));
+      for (int k = 0; k < steps; ++k) LBM_TRY(step());
+      LBM_HIP_TRY(hipEventRecord(ev1, stream));
+      LBM_HIP_TRY(hipEventSynchronize(ev1));
+      float ms = 0.0f;
+      LBM_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+      const double mlups = static_cast<double>(cells()) * steps / (ms * 1e-3) / 1e6;
+      if (mlups > best) {
+        best = mlups;
+        bbx = bx;
+        brows = r;
+      }
+    }
+  block_x = bbx;
+  rows = brows;
+  LBM_HIP_TRY(hipMemcpy(g, backup, bytes, hipMemcpyDeviceToDevice));
+  LBM_HIP_TRY(hipMemcpy(gn, backup, bytes, hipMemcpyDeviceToDevice));
+  LBM_HIP_TRY(hipFree(backup));
+  if (best_bx) *best_bx = bbx;
+  if (best_rows) *best_rows = brows;
+  if (best_mlups) *best_mlups = best;
+  return LBM_OK;
+}
+
+int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
+  if (bytes < 4 || reps < 1) return fail(LBM_EINVAL, "need bytes >= 4 and reps >= 1");
+  const size_t n = bytes / sizeof(float);
+  float *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * sizeof(float)) != hipSuccess) return fail(LBM_ENOMEM, "copy ceiling alloc");
+  if (hipMalloc(&b, n * sizeof(float)) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(LBM_ENOMEM, "copy ceiling alloc");
+  }
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  LBM_HIP_TRY(hipStreamCreate(&st));
+  LBM_HIP_TRY(hipEventCreate(&e0));
+  LBM_HIP_TRY(hipEventCreate(&e1));
+  LBM_HIP_TRY(hipMemsetAsync(a, 0, n * sizeof(float), st));
+  LBM_HIP_TRY(hipMemsetAsync(b, 0, n * sizeof(float), st));
+  int dev = 0, cus = 0;
+  LBM_HIP_TRY(hipGetDevice(&dev));
+  LBM_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  auto best_ms = [&](auto&& launch) -> double {
+    launch();
+    (void)hipStreamSynchronize(st);
+    double best = 1e30;
+    for (int r = 0; r < reps; ++r) {
+      (void)hipEventRecord(e0, st);
+      launch();
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      best = std::min(best, static_cast<double>(ms));
+    }
+    return best;
+  };
+  const double kms = best_ms([&] { (void)launch_copy(a, b, n, cus * 32, 256, st); });
+  LBM_HIP_TRY(hipGetLastError());
+  const double mms = best_ms([&] {
+    (void)hipMemcpyAsync(b, a, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+  });
+  LBM_HIP_TRY(hipStreamSynchronize(st));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  const double total = 2.0 * static_cast<double>(n * sizeof(float));
+  if (copy_gbs) *copy_gbs = total / (kms * 1e-3) / 1e9;
+  if (memcpy_gbs) *memcpy_gbs = total / (mms * 1e-3) / 1e9;
+  return LBM_OK;
+}
+
+}  // namespace lbmhip

@@ -1,0 +1,59 @@
+// State and operations of one plain solver handle (see solver.cpp for the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace lbmhip {
+
+struct SolverState {
+  int nx = 0, ny = 0, prec = 0;
+  double tau = 0.0, fx = 0.0, fy = 0.0;  // as given; converted to the solver real at use
+  std::vector<uint8_t> solid;
+  bool open_x = false;
+  double u_in = 0.0;
+  int block_x = 256, rows = 8;  // launch shape (tile = block_x columns x rows)
+  double fp16_scale = 4096.0;   // stored half = g * scale (LBM_FP16)
+  uint32_t step_count = 0;      // steps taken (seeds the FP16 stochastic rounding)
+
+  void* g = nullptr;   // pull state R, SoA, storage type
+  void* gn = nullptr;  // next state
+  uint16_t* links = nullptr;
+  uint8_t* segflags = nullptr;
+  int nseg = 0;
+  void* force = nullptr;     // per-cell force field (2N reals), or null
+  void* staging = nullptr;   // read-back staging (9N reals)
+  double* red = nullptr;     // reduction partials + result
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  size_t cells() const { return static_cast<size_t>(nx) * ny; }
+  size_t state_bytes() const;  // 9N values of the storage type
+
+  int create(int nx, int ny, double tau, double fx, double fy, const uint8_t* solid, int prec);
+  void release();
+  int set_inflow_outflow(double u);
+  int init_equilibrium(double rho0, double ux0, double uy0);
+  int set_mask(const uint8_t* solid);
+  int set_force_field(const void* fxy);
+  int set_tau(double tau);
+  int set_fp16_scale(double scale);
+  int step();
+  int populations(void* out);
+  int populations_at(int x, int y, void* out9);
+  int set_populations(const void* in);
+  int macroscopic(void* rho, void* ux, void* uy);
+  int reduce(double (&out)[4]);
+  int autotune(int steps, int* bx, int* rows, double* mlups);
+
+ private:
+  int upload_links();
+  int ensure_staging(size_t bytes);
+  size_t staging_bytes = 0;
+};
+
+int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs);
+
+}  // namespace lbmhip

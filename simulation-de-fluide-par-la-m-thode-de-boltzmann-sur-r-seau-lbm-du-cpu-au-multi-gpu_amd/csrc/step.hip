@@ -1,0 +1,232 @@
+// Fused D2Q9 time step for gfx950 (float and double) and its read-back kernels.
+//
+// The hot path of the reference is step_kernel<kPull=true, kNoModulo=true, kOpen=false>
+// (src/v3_cuda_opt/solver_v3.cu:45-113): for every fluid cell, pull the nine populations from
+// the neighbours (halfway bounce-back from solid ones), compute the moments, relax with BGK plus
+// the Guo source, and store to the cell's own slot. This file restates that update for CDNA4:
+//
+//   * A workgroup owns a tile of block_x consecutive columns (a multiple of the 64-wide
+//     wavefront) by rows_per_tile rows; each thread owns one column of the tile and walks its
+//     rows. Row indices are wave-uniform: the row offsets live in SGPRs (buffer soffset) and each
+//     lane only carries three column byte offsets (x - 1, x, x + 1, wrapped once per thread with
+//     a compare, never a modulo).
+//   * Workgroup ids are remapped so that each of the 8 XCDs walks a contiguous band of tiles.
+//   * Solid cells, bounce-back links and open-x edges are handled only in "dirty" 64-column
+//     segments (one scalar flag per wave and row); clean segments, the bulk of a channel, read
+//     nothing but the nine populations: 72 B per cell update in fp32, the algorithmic minimum.
+//   * Per-solver parameters travel as kernel arguments (no __constant__ globals).
+//
+// Floating point: -ffp-contract=off and IEEE division, so float and double steps are
+// bit-identical to the reference's v0.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace lbmhip {
+namespace {
+
+template <class T, bool kOpen, bool kField>
+__global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
+  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
+  const int nx = p.nx;
+  const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+  const bool in_x = x < nx;
+  const unsigned xc = in_x ? x : nx - 1;  // lanes past the edge compute on a valid address
+  constexpr unsigned B = sizeof(T);
+  const Cols cols{(xc == 0 ? nx - 1 : xc - 1) * B, xc * B,
+                  (xc + 1 == static_cast<unsigned>(nx) ? 0 : xc + 1) * B};
+  const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;  // wave-uniform 64-column segment
+  const int y0 = p.row0 + ty * p.rows_per_tile;
+  const int y1 = min(y0 + p.rows_per_tile, p.row0 + p.nrows);
+  Planes g, gn;
+  g.init(p.g, p.N);
+  gn.init(p.gn, p.N);
+  // Force planes (only dereferenced when kField; a null base is never read otherwise).
+  const Rsrc fxr = make_rsrc(p.force), fyr = make_rsrc(p.force + (kField ? p.N : 0));
+
+  for (int y = y0; y < y1; ++y) {
+    int ym = y - 1, yp = y + 1;
+    if (p.wrap_y) {
+      ym = ym < 0 ? p.ny - 1 : ym;
+      yp = yp >= p.ny ? 0 : yp;
+    }
+    const Rows rows{static_cast<unsigned>(ym * nx) * B, static_cast<unsigned>(y * nx) * B,
+                    static_cast<unsigned>(yp * nx) * B};
+    T v[kQ];
+    pull_streamed<T>(g, rows, cols, v);
+    bool active = in_x;
+    if (p.segflags[static_cast<size_t>(y) * p.nseg + seg]) {  // wave-uniform branch
+      const uint32_t link = in_x ? p.links[static_cast<size_t>(y) * nx + xc] : kLinkSolid;
+      active = !(link & kLinkSolid);
+      if (active) apply_links<T, kOpen>(link, g, rows, cols, p.eq_in, v);
+    }
+    if (active) {
+      T fx = p.fx, fy = p.fy;
+      if constexpr (kField) {
+        fx = Stored<T>::load(fxr, cols.c, rows.c);
+        fy = Stored<T>::load(fyr, cols.c, rows.c);
+      }
+      collide(v, fx, fy, p.omega, p.fp);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) Stored<T>::store(gn.r[i], cols.c, rows.c, v[i]);
+    }
+  }
+}
+
+// ---- read-back kernels: one thread per cell, gather with the per-cell link code -------------
+
+template <class T>
+__global__ void gather_kernel(const GatherParams<T> p, T* __restrict__ out) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= p.N) return;
+  const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
+  T v[kQ];
+  if (!gather_cell<T>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in, x, y, v))
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[c * kQ + i] = v[i];
+}
+
+template <class T>
+__global__ void gather_one_kernel(const GatherParams<T> p, int x, int y, T* __restrict__ out) {
+  T v[kQ];
+  if (!gather_cell<T>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in, x, y, v))
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) out[i] = v[i];
+}
+
+template <class T>
+__global__ void macro_kernel(const GatherParams<T> p, T* __restrict__ rho, T* __restrict__ ux,
+                             T* __restrict__ uy) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= p.N) return;
+  const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
+  T v[kQ];
+  T r = T(0), u = T(0), w = T(0);
+  if (gather_cell<T>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in, x, y, v)) {
+    const T fx = p.force ? p.force[c] : p.fx;
+    const T fy = p.force ? p.force[p.N + c] : p.fy;
+    moments(v, fx, fy, r, u, w);
+  }
+  rho[c] = r;
+  ux[c] = u;
+  uy[c] = w;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kReduceThreads) reduce_kernel(const GatherParams<T> p,
+                                                                double* __restrict__ partials) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < p.N;
+       c += static_cast<size_t>(gridDim.x) * kReduceThreads) {
+    const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
+    T v[kQ];
+    if (!gather_cell<T>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in, x, y, v)) continue;
+    s[0] += 1.0;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const double d = static_cast<double>(v[i]);
+      s[1] += d;
+      s[2] += kCx[i] * d;
+      s[3] += kCy[i] * d;
+    }
+  }
+  block_sum4(s);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = s[k];
+}
+
+__global__ void copy_kernel(const float* __restrict__ a, float* __restrict__ b, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kReduceThreads)
+    reduce_final_kernel(const double* __restrict__ partials, int nblocks, double* __restrict__ out) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += kReduceThreads)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += partials[b * 4 + k];
+  block_sum4(s);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = s[k];
+}
+
+template <class T>
+hipError_t launch_step(const StepParams<T>& p, bool open_x, int block_x, hipStream_t stream) {
+  const int tiles_y = (p.nrows + p.rows_per_tile - 1) / p.rows_per_tile;
+  const unsigned grid = static_cast<unsigned>(p.tiles_x) * static_cast<unsigned>(tiles_y);
+  if (grid == 0) return hipSuccess;
+  const bool field = p.force != nullptr;
+  if (open_x) {
+    if (field)
+      step_kernel<T, true, true><<<grid, block_x, 0, stream>>>(p);
+    else
+      step_kernel<T, true, false><<<grid, block_x, 0, stream>>>(p);
+  } else {
+    if (field)
+      step_kernel<T, false, true><<<grid, block_x, 0, stream>>>(p);
+    else
+      step_kernel<T, false, false><<<grid, block_x, 0, stream>>>(p);
+  }
+  return hipGetLastError();
+}
+template hipError_t launch_step<float>(const StepParams<float>&, bool, int, hipStream_t);
+template hipError_t launch_step<double>(const StepParams<double>&, bool, int, hipStream_t);
+
+template <class T>
+hipError_t launch_gather(const GatherParams<T>& p, T* out, hipStream_t stream) {
+  gather_kernel<T><<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, out);
+  return hipGetLastError();
+}
+template hipError_t launch_gather<float>(const GatherParams<float>&, float*, hipStream_t);
+template hipError_t launch_gather<double>(const GatherParams<double>&, double*, hipStream_t);
+
+template <class T>
+hipError_t launch_gather_one(const GatherParams<T>& p, int x, int y, T* out9, hipStream_t stream) {
+  gather_one_kernel<T><<<1, 1, 0, stream>>>(p, x, y, out9);
+  return hipGetLastError();
+}
+template hipError_t launch_gather_one<float>(const GatherParams<float>&, int, int, float*,
+                                             hipStream_t);
+template hipError_t launch_gather_one<double>(const GatherParams<double>&, int, int, double*,
+                                              hipStream_t);
+
+template <class T>
+hipError_t launch_macro(const GatherParams<T>& p, T* rho, T* ux, T* uy, hipStream_t stream) {
+  macro_kernel<T><<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, rho, ux, uy);
+  return hipGetLastError();
+}
+template hipError_t launch_macro<float>(const GatherParams<float>&, float*, float*, float*,
+                                        hipStream_t);
+template hipError_t launch_macro<double>(const GatherParams<double>&, double*, double*, double*,
+                                         hipStream_t);
+
+template <class T>
+hipError_t launch_reduce(const GatherParams<T>& p, double* partials, double* out,
+                         hipStream_t stream) {
+  reduce_kernel<T><<<kReduceBlocks, kReduceThreads, 0, stream>>>(p, partials);
+  reduce_final_kernel<<<1, kReduceThreads, 0, stream>>>(partials, kReduceBlocks, out);
+  return hipGetLastError();
+}
+template hipError_t launch_reduce<float>(const GatherParams<float>&, double*, double*,
+                                         hipStream_t);
+template hipError_t launch_reduce<double>(const GatherParams<double>&, double*, double*,
+                                          hipStream_t);
+
+hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block,
+                       hipStream_t stream) {
+  copy_kernel<<<grid, block, 0, stream>>>(a, b, n);
+  return hipGetLastError();
+}
+
+}  // namespace lbmhip

@@ -1,0 +1,213 @@
+"""GPU parity of the fused step through the C ABI (reference tests/test_v3_vs_v0.cu,
+tests/test_cylinder_v3.cu, tests/poiseuille_check.hpp).
+
+Bar: float and double are bit-identical to the reference's v0 (golden fixtures written by the
+reference itself, and the oracle restatement at the same seeded inputs). Comparisons use the raw
+bit patterns (stricter than the reference's `!=`, which lets +0 and -0 pass)."""
+from __future__ import annotations
+
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_golden, walls_block_mask
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ("float", "double")
+SHAPES = [(64, 1), (64, 3), (128, 8), (192, 5), (256, 8), (512, 2), (1024, 1)]
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+def assert_bitwise(got, ref, what=""):
+    diff = np.flatnonzero(bits(got) != bits(ref))
+    assert diff.size == 0, (f"{what}: {diff.size} differing values, first at cell {diff[0] // 9} "
+                            f"direction {diff[0] % 9}: {got[diff[0]]!r} vs {ref[diff[0]]!r}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_walls_block_matches_reference_v0(lbm, prec, shape):
+    """53x31, 500 steps, walls + block, init (1, 0.02, 0.01) (test_v3_vs_v0.cu:20-37)."""
+    solid, ref, p = load_golden(f"walls_block_53x31_500_{prec}.npz")
+    s = lbm.Solver(53, 31, p["tau"], p["fx"], p["fy"], solid, prec)
+    s.set_block(*shape)
+    s.init_equilibrium(*p["init"])
+    s.run(p["steps"])
+    assert_bitwise(s.populations(), ref, f"{prec} block {shape}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("shape", [(64, 1), (128, 4), (256, 8)])
+def test_cylinder_open_x_matches_reference_v0(lbm, prec, shape):
+    """120x48 cylinder D = 10, inlet/outlet, 1000 steps (tests/cylinder_case.hpp:16-31)."""
+    solid, ref, p = load_golden(f"cylinder_120x48_1000_{prec}.npz")
+    s = lbm.Solver(p["nx"], p["ny"], p["tau"], 0.0, 0.0, solid, prec)
+    s.set_block(*shape)
+    s.set_inflow_outflow(p["u_in"])
+    s.init_equilibrium(*p["init"])
+    s.run(p["steps"])
+    assert_bitwise(s.populations(), ref, f"cylinder {prec} {shape}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("seed", (0, 1, 2))
+def test_seeded_block_rest_init_matches_reference_v0(lbm, prec, seed):
+    """53x31, rest init, 8x8 block placed by seed (validation instance of the plain method)."""
+    solid, ref, p = load_golden(f"seeded_block_53x31_500_seed{seed}_{prec}.npz")
+    s = lbm.Solver(53, 31, p["tau"], p["fx"], p["fy"], solid, prec)
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    s.run(p["steps"])
+    assert_bitwise(s.populations(), ref, f"seed {seed} {prec}")
+
+
+def test_crop256_double_matches_reference_hash(lbm):
+    """256x256 walled channel, tau 0.6, gx 1e-6, 200 steps, double: SHA-256 of the reference
+    v0 output (the fp64 crop of the 8192^2 instance)."""
+    import hashlib
+
+    h = json.loads((GOLDEN / "hashes.json").read_text())["crop256_f64"]
+    s = lbm.Solver(256, 256, h["tau"], h["fx"], h["fy"], lbm.make_channel(256, 256), "double")
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    s.run(h["steps"])
+    S = s.populations()
+    np.testing.assert_array_equal(S[h["sample_index"]], np.array(h["sample_value"]))
+    assert hashlib.sha256(S.tobytes()).hexdigest() == h["sha256"]
+    assert abs(s.total_mass() - h["total_mass"]) <= 1e-12 * h["total_mass"]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_macroscopic_and_reductions_match_oracle(lbm, oracle_mod, prec):
+    solid = walls_block_mask()
+    s = lbm.Solver(53, 31, 0.8, 1e-5, 2e-6, solid, prec)
+    o = oracle_mod.Oracle(53, 31, 0.8, 1e-5, 2e-6, solid, prec, "v0")
+    for x in (s, o):
+        x.init_equilibrium(1.0, 0.02, 0.01)
+        x.run(137)
+    for a, b in zip(s.macroscopic(), o.macroscopic()):
+        assert_bitwise(a, b, "macroscopic")
+    m_gpu, m_ref = s.total_mass(), o.total_mass()
+    assert abs(m_gpu - m_ref) <= 1e-12 * m_ref
+    (px, py), (qx, qy) = s.momentum(), o.momentum()
+    assert abs(px - qx) <= 1e-12 * (abs(qx) + 1e-300) + 1e-15
+    assert abs(py - qy) <= 1e-12 * (abs(qy) + 1e-300) + 1e-15
+    S = s.populations().reshape(-1, 9)
+    for (x, y) in ((0, 1), (5, 9), (52, 29), (20, 15), (12, 10)):
+        assert_bitwise(s.populations_at(x, y), S[y * 53 + x], f"populations_at {x},{y}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_set_populations_round_trip(lbm, prec):
+    """Upload an arbitrary post-streaming state: its preimage gathers back to the same values."""
+    solid = walls_block_mask()
+    rng = np.random.default_rng(7)
+    dt = np.float32 if prec == "float" else np.float64
+    S = (rng.standard_normal(53 * 31 * 9) * 1e-3).astype(dt)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    s = lbm.Solver(53, 31, 0.8, 1e-5, 2e-6, solid, prec)
+    s.set_populations(S)
+    assert_bitwise(s.populations(), S, "round trip")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_random_state_and_force_field_match_oracle(lbm, oracle_mod, prec):
+    """Arbitrary initial state and a per-cell force field (extension used by the adjoint
+    form), 200 steps, against the oracle with the same field."""
+    solid = walls_block_mask()
+    rng = np.random.default_rng(11)
+    dt = np.float32 if prec == "float" else np.float64
+    S = (rng.standard_normal(53 * 31 * 9) * 1e-3).astype(dt)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    F = (rng.standard_normal(2 * 53 * 31) * 1e-5).astype(dt)
+    s = lbm.Solver(53, 31, 0.7, 0.0, 0.0, solid, prec)
+    o = oracle_mod.Oracle(53, 31, 0.7, 0.0, 0.0, solid, prec, "v0")
+    for x in (s, o):
+        x.set_populations(S)
+        x.set_force_field(F)
+        x.run(200)
+    assert_bitwise(s.populations(), o.populations(), "force field")
+    for a, b in zip(s.macroscopic(), o.macroscopic()):
+        assert_bitwise(a, b, "macroscopic with field")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_uniform_field_equals_uniform_force(lbm, prec):
+    solid = walls_block_mask()
+    N = 53 * 31
+    dt = np.float32 if prec == "float" else np.float64
+    F = np.concatenate([np.full(N, 1e-5), np.full(N, 2e-6)]).astype(dt)
+    a = lbm.Solver(53, 31, 0.8, 1e-5, 2e-6, solid, prec)
+    b = lbm.Solver(53, 31, 0.8, 0.0, 0.0, solid, prec)
+    b.set_force_field(F)
+    for x in (a, b):
+        x.init_equilibrium(1.0, 0.02, 0.01)
+        x.run(300)
+    assert_bitwise(a.populations(), b.populations(), "uniform field")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_setters_match_oracle_restart(lbm, oracle_mod, prec):
+    """set_tau / set_force / set_mask between runs == an oracle restarted from the state."""
+    solid = walls_block_mask()
+    s = lbm.Solver(53, 31, 0.8, 1e-5, 2e-6, solid, prec)
+    o = oracle_mod.Oracle(53, 31, 0.8, 1e-5, 2e-6, solid, prec, "v0")
+    for x in (s, o):
+        x.init_equilibrium(1.0, 0.02, 0.01)
+        x.run(100)
+    S = o.populations()
+    solid2 = solid.copy().reshape(31, 53)
+    solid2[20:24, 30:34] = 1
+    solid2 = solid2.reshape(-1)
+    S.reshape(-1, 9)[solid2.astype(bool)] = 0
+    s.set_populations(S)
+    s.set_tau(0.65)
+    s.set_force(-3e-6, 1e-6)
+    s.set_mask(solid2)
+    s.set_populations(S)
+    s.run(150)
+    o2 = oracle_mod.Oracle(53, 31, 0.65, -3e-6, 1e-6, solid2, prec, "v0")
+    o2.set_populations(S)
+    o2.run(150)
+    assert_bitwise(s.populations(), o2.populations(), "setters")
+
+
+def test_channel_8192_columns_match_oracle_narrow_channel(lbm, oracle_mod):
+    """Full-size instance, 8192 x 8192 walled channel, tau 0.6, gx 1e-6, 200 steps (float):
+    the flow is x-invariant, so every column must equal, bit for bit, the oracle's 4 x 8192
+    channel with the same rows. A size-independent property checked at the real size."""
+    nx = ny = 8192
+    s = lbm.Solver(nx, ny, 0.6, 1e-6, 0.0, lbm.make_channel(nx, ny), "float")
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    s.run(200)
+    rho, ux, uy = (a.reshape(ny, nx) for a in s.macroscopic())
+    o = oracle_mod.Oracle(4, ny, 0.6, 1e-6, 0.0, oracle_mod.make_channel(4, ny), "float", "v0")
+    o.init_equilibrium(1.0, 0.0, 0.0)
+    o.run(200)
+    r0, u0, v0 = (a.reshape(ny, 4)[:, :1] for a in o.macroscopic())
+    for got, ref, name in ((rho, r0, "rho"), (ux, u0, "ux"), (uy, v0, "uy")):
+        bad = np.count_nonzero(bits(got) != bits(np.broadcast_to(ref, got.shape)))
+        assert bad == 0, f"{name}: {bad} cells differ"
+    assert ux[ny // 2, 0] > 0
+    m0 = float(nx) * (ny - 2)
+    assert abs(s.total_mass() - m0) / m0 < 1e-6
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_non_multiple_grid_and_tiny_grid(lbm, oracle_mod, prec):
+    """Ragged sizes: nx not a multiple of 64, tiles cut in y, a 1-row-thick fluid layer."""
+    for nx, ny in ((67, 9), (130, 5), (3, 3), (1, 5)):
+        solid = np.zeros(nx * ny, np.uint8)
+        solid[:nx] = 1
+        s = lbm.Solver(nx, ny, 0.9, 2e-5, -1e-5, solid, prec)
+        s.set_block(64, 4)
+        o = oracle_mod.Oracle(nx, ny, 0.9, 2e-5, -1e-5, solid, prec, "v0")
+        for x in (s, o):
+            x.init_equilibrium(1.0, 0.01, -0.02)
+            x.run(37)
+        assert_bitwise(s.populations(), o.populations(), f"{nx}x{ny}")
