@@ -31,6 +31,13 @@ _lib = None
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
+class DomainDesc(C.Structure):
+    """lbm_domain_desc (include/lbm_hip.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("tau", C.c_double), ("force_x", C.c_double),
+                ("force_y", C.c_double), ("solid", C.POINTER(C.c_uint8)), ("rho0", C.c_double),
+                ("ux0", C.c_double), ("uy0", C.c_double)]
+
+
 def library_path() -> pathlib.Path:
     return LIB_PATH
 
@@ -83,6 +90,33 @@ def _declare(L):
         "lbm_autotune": ([vp, i, ip, ip, dp], i),
         "lbm_poiseuille_force": ([i, i, d, d], d),
         "lbm_copy_ceiling": ([sz, i, dp, dp], i),
+        "lbm_strip_create": ([i, i, i, C.POINTER(vp)], i),
+        "lbm_strip_destroy": ([vp], i),
+        "lbm_strip_set_params": ([vp, d, d, d, i, d], i),
+        "lbm_strip_upload_solid": ([vp, _u8p], i),
+        "lbm_strip_upload_state": ([vp, vp], i),
+        "lbm_strip_download_state": ([vp, vp], i),
+        "lbm_strip_compute_rows": ([vp, i, i, i], i),
+        "lbm_strip_pack_halo": ([vp, vp, vp], i),
+        "lbm_strip_unpack_halo": ([vp, vp, vp], i),
+        "lbm_strip_finish_step": ([vp], i),
+        "lbm_strip_set_block": ([vp, i, i], i),
+        "lbm_strip_timed_steps": ([vp, i, dp], i),
+        "lbm_batch_create": ([i, C.POINTER(DomainDesc), i, C.POINTER(vp)], i),
+        "lbm_batch_destroy": ([vp], i),
+        "lbm_batch_set_graph_steps": ([vp, i], i),
+        "lbm_batch_set_shape": ([vp, i, i, i], i),
+        "lbm_batch_run": ([vp, i], i),
+        "lbm_batch_timed_run": ([vp, i, dp], i),
+        "lbm_batch_populations": ([vp, i, vp], i),
+        "lbm_batch_total_cells": ([vp, llp], i),
+        "lbm_batch_autotune": ([vp, i, ip, ip, ip, dp], i),
+        "lbm_adjoint_create": ([i, i, d, _u8p, i, i, i, i, i, i, i, i, C.POINTER(vp)], i),
+        "lbm_adjoint_destroy": ([vp], i),
+        "lbm_adjoint_set_force_field": ([vp, vp], i),
+        "lbm_adjoint_set_tau": ([vp, d], i),
+        "lbm_adjoint_forward": ([vp, dp, dp], i),
+        "lbm_adjoint_gradient": ([vp, dp, vp, dp, dp, llp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -324,6 +358,232 @@ class Solver:
         bx, r, m = C.c_int(), C.c_int(), C.c_double()
         check(lib().lbm_autotune(self._h, int(steps), C.byref(bx), C.byref(r), C.byref(m)))
         return bx.value, r.value, m.value
+
+
+# ---- strip engine (one GPU's share of the v4 decomposition) ---------------------------------
+
+HALO_UP = (2, 5, 6)    # c_y = +1: top owned row -> ghost row 0 of the strip above
+HALO_DOWN = (4, 7, 8)  # c_y = -1: bottom owned row -> top ghost row of the strip below
+
+
+class Strip:
+    """StripEngine's interface (src/v4_mpi_cuda/engine.hpp:36-61) on the fused step: ny_local
+    owned rows plus ghost rows 0 and ny_local + 1, SoA state of 9 * nx * (ny_local + 2) reals."""
+
+    def __init__(self, nx, ny_local, precision="float"):
+        self.precision = precision_code(precision)
+        self.dtype = real_dtype(self.precision)
+        self.nx, self.ny_local = int(nx), int(ny_local)
+        h = C.c_void_p()
+        check(lib().lbm_strip_create(self.nx, self.ny_local, self.precision, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.lbm_strip_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def N(self):
+        return self.nx * (self.ny_local + 2)
+
+    def set_params(self, tau, fx=0.0, fy=0.0, open_x=False, u_in=0.0):
+        check(lib().lbm_strip_set_params(self._h, float(tau), float(fx), float(fy), int(open_x),
+                                         float(u_in)))
+
+    def upload_solid(self, solid_strip):
+        m = np.ascontiguousarray(solid_strip, dtype=np.uint8).reshape(-1)
+        if m.size != self.N:
+            raise ValueError("strip mask must hold nx * (ny_local + 2) bytes")
+        check(lib().lbm_strip_upload_solid(self._h, m))
+
+    def upload_state(self, g):
+        a = np.ascontiguousarray(g, dtype=self.dtype).reshape(-1)
+        if a.size != 9 * self.N:
+            raise ValueError("strip state must hold 9 * nx * (ny_local + 2) values")
+        check(lib().lbm_strip_upload_state(self._h, a.ctypes.data))
+
+    def download_state(self):
+        out = np.empty(9 * self.N, dtype=self.dtype)
+        check(lib().lbm_strip_download_state(self._h, out.ctypes.data))
+        return out
+
+    def compute_rows(self, first, last, stream=0):
+        check(lib().lbm_strip_compute_rows(self._h, int(first), int(last), int(stream)))
+
+    def pack_halo(self):
+        up = np.empty(3 * self.nx, dtype=self.dtype)
+        down = np.empty(3 * self.nx, dtype=self.dtype)
+        check(lib().lbm_strip_pack_halo(self._h, up.ctypes.data, down.ctypes.data))
+        return up, down
+
+    def unpack_halo(self, recv_from_down, recv_from_up):
+        a = np.ascontiguousarray(recv_from_down, dtype=self.dtype)
+        b = np.ascontiguousarray(recv_from_up, dtype=self.dtype)
+        check(lib().lbm_strip_unpack_halo(self._h, a.ctypes.data, b.ctypes.data))
+
+    def finish_step(self):
+        check(lib().lbm_strip_finish_step(self._h))
+
+    def set_block(self, block_x, rows_per_thread):
+        check(lib().lbm_strip_set_block(self._h, int(block_x), int(rows_per_thread)))
+
+    def timed_steps(self, steps) -> float:
+        sec = C.c_double()
+        check(lib().lbm_strip_timed_steps(self._h, int(steps), C.byref(sec)))
+        return sec.value
+
+
+# ---- batched persistent step -----------------------------------------------------------------
+
+def random_domains(seed: int = 0, n: int = 256, nx_range=(32, 1024), ny_range=(32, 512),
+                   tau_range=(0.55, 1.5), force_range=(1e-6, 1e-4), max_obstacles=4,
+                   radius_range=(4, 24)):
+    """Independent walled channels drawn from `seed`: nx, ny uniform integers in the given
+    ranges (inclusive), tau uniform, gx and gy log-uniform, 0..max_obstacles obstacles (squares
+    or discs of radius radius_range) at uniform positions. Returns dicts with nx, ny, tau, fx,
+    fy, solid (flat uint8)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(force_range[0]), np.log(force_range[1])
+    out = []
+    for _ in range(n):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        tau = float(rng.uniform(*tau_range))
+        fx, fy = (float(np.exp(rng.uniform(lo, hi))) for _ in range(2))
+        m = make_channel(nx, ny).reshape(ny, nx)
+        for _ in range(int(rng.integers(0, max_obstacles + 1))):
+            r = int(rng.integers(radius_range[0], radius_range[1] + 1))
+            xc, yc = float(rng.uniform(0, nx)), float(rng.uniform(0, ny))
+            if rng.integers(0, 2):
+                x0, x1 = max(0, int(xc) - r), min(nx, int(xc) + r)
+                y0, y1 = max(0, int(yc) - r), min(ny, int(yc) + r)
+                m[y0:y1, x0:x1] = 1
+            else:
+                flat = m.reshape(-1)
+                add_disc(flat, nx, ny, xc, yc, r)
+        out.append(dict(nx=nx, ny=ny, tau=tau, fx=fx, fy=fy, solid=m.reshape(-1).copy()))
+    return out
+
+
+class Batch:
+    """Many independent domains stepped by one persistent kernel per step, in captured HIP
+    graphs of `graph_steps` steps. Every domain is bit-identical to a single-domain Solver."""
+
+    def __init__(self, domains, precision="float", init=(1.0, 0.0, 0.0)):
+        self.precision = precision_code(precision)
+        self.dtype = real_dtype(self.precision)
+        self.domains = list(domains)
+        self._masks = [np.ascontiguousarray(d["solid"], dtype=np.uint8).reshape(-1)
+                       for d in self.domains]
+        descs = (DomainDesc * len(self.domains))()
+        for k, (d, m) in enumerate(zip(self.domains, self._masks)):
+            if m.size != d["nx"] * d["ny"]:
+                raise ValueError(f"domain {k}: solid mask size does not match the grid")
+            descs[k] = DomainDesc(d["nx"], d["ny"], d["tau"], d["fx"], d["fy"],
+                                  m.ctypes.data_as(C.POINTER(C.c_uint8)), *init)
+        h = C.c_void_p()
+        check(lib().lbm_batch_create(len(self.domains), descs, self.precision, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.lbm_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_graph_steps(self, k):
+        check(lib().lbm_batch_set_graph_steps(self._h, int(k)))
+
+    def set_shape(self, tile_x, tile_rows, workgroups):
+        check(lib().lbm_batch_set_shape(self._h, int(tile_x), int(tile_rows), int(workgroups)))
+
+    def run(self, steps):
+        check(lib().lbm_batch_run(self._h, int(steps)))
+
+    def timed_run(self, steps) -> float:
+        s = C.c_double()
+        check(lib().lbm_batch_timed_run(self._h, int(steps), C.byref(s)))
+        return s.value
+
+    def populations(self, k) -> np.ndarray:
+        d = self.domains[k]
+        out = np.empty(9 * d["nx"] * d["ny"], dtype=self.dtype)
+        check(lib().lbm_batch_populations(self._h, int(k), out.ctypes.data))
+        return out
+
+    def total_cells(self) -> int:
+        n = C.c_longlong()
+        check(lib().lbm_batch_total_cells(self._h, C.byref(n)))
+        return n.value
+
+    def autotune(self, steps=10):
+        tx, tr, wg, m = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().lbm_batch_autotune(self._h, int(steps), C.byref(tx), C.byref(tr), C.byref(wg),
+                                       C.byref(m)))
+        return tx.value, tr.value, wg.value, m.value
+
+
+# ---- discrete adjoint ------------------------------------------------------------------------
+
+class Adjoint:
+    """Gradient of J = mean u_x over a probe window, averaged over the last `avg_steps` states of
+    a `steps`-step run from rest, w.r.t. a per-cell force field and tau (discrete adjoint with
+    binomial checkpointing in `snapshots` stored states)."""
+
+    def __init__(self, nx, ny, tau, solid, steps, probe, avg_steps, snapshots,
+                 precision="float"):
+        self.precision = precision_code(precision)
+        self.dtype = real_dtype(self.precision)
+        self.nx, self.ny = int(nx), int(ny)
+        self._solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        if self._solid.size != self.nx * self.ny:
+            raise ValueError("solid mask size does not match the grid")
+        x0, y0, w, h = (int(v) for v in probe)
+        hd = C.c_void_p()
+        check(lib().lbm_adjoint_create(self.nx, self.ny, float(tau), self._solid, int(steps), x0,
+                                       y0, w, h, int(avg_steps), int(snapshots), self.precision,
+                                       C.byref(hd)))
+        self._h = hd
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.lbm_adjoint_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_force_field(self, fxy):
+        a = np.ascontiguousarray(fxy, dtype=self.dtype).reshape(-1)
+        if a.size != 2 * self.nx * self.ny:
+            raise ValueError("force field must hold 2 * nx * ny values")
+        check(lib().lbm_adjoint_set_force_field(self._h, a.ctypes.data))
+
+    def set_tau(self, tau):
+        check(lib().lbm_adjoint_set_tau(self._h, float(tau)))
+
+    def forward(self):
+        J, s = C.c_double(), C.c_double()
+        check(lib().lbm_adjoint_forward(self._h, C.byref(J), C.byref(s)))
+        return J.value, s.value
+
+    def gradient(self):
+        """Returns (J, dJ/dF (2, ny, nx), dJ/dtau, seconds, forward steps recomputed)."""
+        J, dt, s = C.c_double(), C.c_double(), C.c_double()
+        fs = C.c_longlong()
+        dF = np.empty(2 * self.nx * self.ny, dtype=self.dtype)
+        check(lib().lbm_adjoint_gradient(self._h, C.byref(J), dF.ctypes.data, C.byref(dt),
+                                         C.byref(s), C.byref(fs)))
+        return J.value, dF.reshape(2, self.ny, self.nx), dt.value, s.value, fs.value
 
 
 # ---- analysis (scripts/strouhal.py:57-81 method) ---------------------------------------------

@@ -23,8 +23,6 @@ CSRC = HERE / "csrc"
 OBJ = HERE / "build"
 LIB = HERE / "liblbm_hip.so"
 
-SOURCES = ["step.hip", "fp16.hip", "strip.hip", "batch.hip", "adjoint.hip",
-           "host_common.cpp", "solver_impl.cpp", "solver.cpp"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 
@@ -57,7 +55,7 @@ def compile_one(src: str) -> pathlib.Path:
 
 def build(jobs: int | None = None) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
-    srcs = [s for s in SOURCES if (CSRC / s).exists()]
+    srcs = sorted(p.name for p in CSRC.iterdir() if p.suffix in (".hip", ".cpp"))
     with cf.ThreadPoolExecutor(max_workers=jobs or min(8, os.cpu_count() or 4)) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -1,0 +1,423 @@
+// C ABI of the batched persistent step (lbm_batch_*): host-side layout of the domains, the
+// descriptor table and tile work-list, HIP-graph capture of multi-step loops and the autotuner
+// of the launch shape. No reference counterpart.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "d2q9.hpp"
+#include "host_common.hpp"
+#include "kernels.hpp"
+#include "lbm_hip.h"
+
+using namespace lbmhip;
+
+namespace {
+
+struct DomainHost {
+  int nx, ny;
+  double tau, fx, fy, rho0, ux0, uy0;
+  std::vector<uint8_t> solid;
+  size_t offset = 0, cell_offset = 0, seg_offset = 0;
+  int nseg = 0;
+  size_t N() const { return static_cast<size_t>(nx) * ny; }
+};
+
+constexpr size_t kAlignElems = 64;  // plane-0 offsets aligned to 64 elements
+
+}  // namespace
+
+struct lbm_batch {
+  int prec = LBM_FLOAT;
+  std::vector<DomainHost> doms;
+  size_t total_elems = 0, total_cells = 0, total_segs = 0;
+  void* g = nullptr;
+  void* gn = nullptr;
+  uint16_t* links = nullptr;
+  uint8_t* segflags = nullptr;
+  void* ddesc = nullptr;
+  BatchTile* dtiles = nullptr;
+  size_t tiles_cap = 0;
+  int ntiles = 0;
+  int tile_x = 256, tile_rows = 4, workgroups = 0;
+  int graph_steps = 100;
+  int parity = 0;  // 0: current state in g
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
+
+  size_t rs() const { return real_size(prec); }
+
+  void drop_graphs() {
+    for (auto& e : exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
+
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    for (void* p : {g, gn, static_cast<void*>(links), static_cast<void*>(segflags), ddesc,
+                    static_cast<void*>(dtiles), staging})
+      if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <class T>
+  int upload_descriptors() {
+    std::vector<BatchDomain<T>> d(doms.size());
+    for (size_t k = 0; k < doms.size(); ++k) {
+      const DomainHost& h = doms[k];
+      BatchDomain<T>& b = d[k];
+      b.offset = h.offset;
+      b.cell_offset = h.cell_offset;
+      b.seg_offset = h.seg_offset;
+      b.N = h.N();
+      b.nx = h.nx;
+      b.ny = h.ny;
+      b.nseg = h.nseg;
+      b.pad = 0;
+      const T tau = static_cast<T>(h.tau);
+      b.omega = T(1) / tau;  // as the single-domain step (solver_v3.cu:252-253)
+      b.fp = T(1) - T(0.5) * b.omega;
+      b.fx = static_cast<T>(h.fx);
+      b.fy = static_cast<T>(h.fy);
+    }
+    LBM_HIP_TRY(hipMemcpy(ddesc, d.data(), d.size() * sizeof(BatchDomain<T>), hipMemcpyHostToDevice));
+    return LBM_OK;
+  }
+
+  int build_tiles() {
+    std::vector<BatchTile> t;
+    for (size_t k = 0; k < doms.size(); ++k) {
+      const int tx = (doms[k].nx + tile_x - 1) / tile_x, ty = (doms[k].ny + tile_rows - 1) / tile_rows;
+      for (int y = 0; y < ty; ++y)
+        for (int x = 0; x < tx; ++x) t.push_back(BatchTile{static_cast<int>(k), x, y, 0});
+    }
+    ntiles = static_cast<int>(t.size());
+    if (t.size() > tiles_cap) {
+      if (dtiles) LBM_HIP_TRY(hipFree(dtiles));
+      dtiles = nullptr;
+      if (hipMalloc(&dtiles, t.size() * sizeof(BatchTile)) != hipSuccess)
+        return fail(LBM_ENOMEM, "device allocation of the tile list failed");
+      tiles_cap = t.size();
+    }
+    LBM_HIP_TRY(hipMemcpy(dtiles, t.data(), t.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
+    drop_graphs();
+    return LBM_OK;
+  }
+
+  template <class T>
+  BatchParams<T> params(bool flip) const {
+    BatchParams<T> p{};
+    p.g = static_cast<const T*>(flip ? gn : g);
+    p.gn = static_cast<T*>(flip ? g : gn);
+    p.links = links;
+    p.segflags = segflags;
+    p.domains = static_cast<const BatchDomain<T>*>(ddesc);
+    p.tiles = dtiles;
+    p.ntiles = ntiles;
+    p.tile_rows = tile_rows;
+    const int wg = std::max(1, std::min(workgroups, ntiles));
+    p.chunk = (ntiles + wg - 1) / wg;
+    return p;
+  }
+
+  int launch(bool flip) {
+    const int wg = std::max(1, std::min(workgroups, ntiles));
+    const hipError_t e = prec == LBM_DOUBLE
+                             ? launch_batch(params<double>(flip), tile_x, wg, stream)
+                             : launch_batch(params<float>(flip), tile_x, wg, stream);
+    if (e != hipSuccess) return fail(LBM_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
+    return LBM_OK;
+  }
+
+  int step_direct() {
+    LBM_TRY(launch(parity == 1));
+    parity ^= 1;
+    return LBM_OK;
+  }
+
+  // A graph of graph_steps steps starting from parity p; launches are captured, not run.
+  int build_graph(int p) {
+    hipGraph_t graph = nullptr;
+    LBM_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int par = p;
+    int rc = LBM_OK;
+    for (int k = 0; k < graph_steps && rc == LBM_OK; ++k) {
+      rc = launch(par == 1);
+      par ^= 1;
+    }
+    const hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (rc != LBM_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(LBM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&exec[p], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return fail(LBM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e2));
+    return LBM_OK;
+  }
+
+  int run_async(int steps) {
+    while (steps >= graph_steps && graph_steps > 1) {
+      if (!exec[parity]) LBM_TRY(build_graph(parity));
+      LBM_HIP_TRY(hipGraphLaunch(exec[parity], stream));
+      parity ^= graph_steps & 1;
+      steps -= graph_steps;
+    }
+    for (int k = 0; k < steps; ++k) LBM_TRY(step_direct());
+    return LBM_OK;
+  }
+
+  void* current() const { return parity == 0 ? g : gn; }
+};
+
+extern "C" {
+
+int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_batch** out) {
+  if (!out) return fail(LBM_EINVAL, "out handle pointer is null");
+  *out = nullptr;
+  if (n < 1 || !domains) return fail(LBM_EINVAL, "need at least one domain");
+  if (precision != LBM_FLOAT && precision != LBM_DOUBLE)
+    return fail(LBM_EINVAL, "the batched step runs in LBM_FLOAT or LBM_DOUBLE");
+  auto* b = new lbm_batch();
+  b->prec = precision;
+  auto bail = [&](int rc) {
+    b->release();
+    delete b;
+    return rc;
+  };
+  b->doms.resize(n);
+  for (int k = 0; k < n; ++k) {
+    const lbm_domain_desc& d = domains[k];
+    if (d.nx < 1 || d.ny < 1 || !d.solid) return bail(fail(LBM_EINVAL, "domain " + std::to_string(k) + ": invalid grid or mask"));
+    const bool ok = precision == LBM_DOUBLE ? d.tau > 0.5 : static_cast<float>(d.tau) > 0.5f;
+    if (!ok) return bail(fail(LBM_EINVAL, "domain " + std::to_string(k) + ": tau must be > 0.5"));
+    DomainHost& h = b->doms[k];
+    h.nx = d.nx;
+    h.ny = d.ny;
+    h.tau = d.tau;
+    h.fx = d.force_x;
+    h.fy = d.force_y;
+    h.rho0 = d.rho0;
+    h.ux0 = d.ux0;
+    h.uy0 = d.uy0;
+    h.solid.assign(d.solid, d.solid + h.N());
+    h.nseg = (d.nx + 63) / 64;
+    h.offset = b->total_elems;
+    h.cell_offset = b->total_cells;
+    h.seg_offset = b->total_segs;
+    b->total_elems += (kQ * h.N() + kAlignElems - 1) / kAlignElems * kAlignElems;
+    b->total_cells += h.N();
+    b->total_segs += static_cast<size_t>(h.ny) * h.nseg;
+  }
+  int rc = parse_block_env(&b->tile_x, &b->tile_rows);
+  if (rc != LBM_OK) return bail(rc);
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(LBM_EHIP, "stream / event creation failed"));
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->workgroups = std::max(1, cus) * 8;
+  const size_t bytes = b->total_elems * b->rs();
+  if (hipMalloc(&b->g, bytes) != hipSuccess || hipMalloc(&b->gn, bytes) != hipSuccess ||
+      hipMalloc(&b->links, b->total_cells * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&b->segflags, b->total_segs) != hipSuccess ||
+      hipMalloc(&b->ddesc, n * std::max(sizeof(BatchDomain<double>), sizeof(BatchDomain<float>))) != hipSuccess)
+    return bail(fail(LBM_ENOMEM, "device allocation of the batch failed"));
+  // link codes, flags and the initial state of every domain
+  std::vector<uint16_t> codes(b->total_cells);
+  std::vector<uint8_t> segs(b->total_segs);
+  std::vector<char> state(bytes, 0);
+  for (auto& h : b->doms) {
+    const Links L = build_links(h.nx, h.ny, h.solid.data(), false, true);
+    std::copy(L.code.begin(), L.code.end(), codes.begin() + h.cell_offset);
+    std::copy(L.seg.begin(), L.seg.end(), segs.begin() + h.seg_offset);
+    if (precision == LBM_DOUBLE) {
+      double eq[kQ];
+      equilibrium_state<double>(h.rho0, h.ux0, h.uy0, eq);
+      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq,
+                       reinterpret_cast<double*>(state.data()) + h.offset);
+    } else {
+      float eq[kQ];
+      equilibrium_state<float>(static_cast<float>(h.rho0), static_cast<float>(h.ux0),
+                               static_cast<float>(h.uy0), eq);
+      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq,
+                       reinterpret_cast<float*>(state.data()) + h.offset);
+    }
+  }
+  if (hipMemcpy(b->links, codes.data(), codes.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->segflags, segs.data(), segs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->g, state.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->gn, state.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(LBM_EHIP, "upload of the batch failed"));
+  rc = precision == LBM_DOUBLE ? b->upload_descriptors<double>() : b->upload_descriptors<float>();
+  if (rc == LBM_OK) rc = b->build_tiles();
+  if (rc != LBM_OK) return bail(rc);
+  *out = b;
+  return LBM_OK;
+}
+
+int lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return LBM_OK;
+  b->release();
+  delete b;
+  return LBM_OK;
+}
+
+int lbm_batch_set_graph_steps(lbm_batch* b, int steps_per_graph) {
+  if (!b || steps_per_graph < 1) return fail(LBM_EINVAL, "need a batch and steps_per_graph >= 1");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->graph_steps = steps_per_graph;
+  b->drop_graphs();
+  return LBM_OK;
+}
+
+int lbm_batch_set_shape(lbm_batch* b, int tile_x, int tile_rows, int workgroups) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  LBM_TRY(check_block(tile_x, tile_rows));
+  if (workgroups < 1) return fail(LBM_EINVAL, "workgroups must be >= 1");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tile_x = tile_x;
+  b->tile_rows = tile_rows;
+  b->workgroups = workgroups;
+  return b->build_tiles();
+}
+
+int lbm_batch_run(lbm_batch* b, int steps) {
+  if (!b || steps < 0) return fail(LBM_EINVAL, "null batch handle or negative steps");
+  LBM_TRY(b->run_async(steps));
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  return LBM_OK;
+}
+
+int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds) {
+  if (!b || steps < 0) return fail(LBM_EINVAL, "null batch handle or negative steps");
+  // Graphs are built before the timed region (instantiation is not part of a step).
+  if (steps >= b->graph_steps && b->graph_steps > 1) {
+    if (!b->exec[b->parity]) LBM_TRY(b->build_graph(b->parity));
+    if ((b->graph_steps & 1) && !b->exec[b->parity ^ 1]) LBM_TRY(b->build_graph(b->parity ^ 1));
+  }
+  LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  LBM_TRY(b->run_async(steps));
+  LBM_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  LBM_HIP_TRY(hipEventSynchronize(b->ev1));
+  float ms = 0.0f;
+  LBM_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (seconds) *seconds = ms * 1e-3;
+  return LBM_OK;
+}
+
+int lbm_batch_populations(lbm_batch* b, int domain, void* out) {
+  if (!b || !out || domain < 0 || domain >= static_cast<int>(b->doms.size()))
+    return fail(LBM_EINVAL, "invalid batch handle, domain or output");
+  const DomainHost& h = b->doms[domain];
+  const size_t bytes = kQ * h.N() * b->rs();
+  if (b->staging_bytes < bytes) {
+    if (b->staging) LBM_HIP_TRY(hipFree(b->staging));
+    b->staging = nullptr;
+    b->staging_bytes = 0;
+    if (hipMalloc(&b->staging, bytes) != hipSuccess) return fail(LBM_ENOMEM, "staging allocation failed");
+    b->staging_bytes = bytes;
+  }
+  hipError_t e;
+  if (b->prec == LBM_DOUBLE) {
+    GatherParams<double> p{};
+    p.g = static_cast<const double*>(b->current()) + h.offset;
+    p.links = b->links + h.cell_offset;
+    p.N = h.N();
+    p.nx = h.nx;
+    p.ny = h.ny;
+    p.fx = h.fx;
+    p.fy = h.fy;
+    e = launch_gather(p, static_cast<double*>(b->staging), b->stream);
+  } else {
+    GatherParams<float> p{};
+    p.g = static_cast<const float*>(b->current()) + h.offset;
+    p.links = b->links + h.cell_offset;
+    p.N = h.N();
+    p.nx = h.nx;
+    p.ny = h.ny;
+    p.fx = static_cast<float>(h.fx);
+    p.fy = static_cast<float>(h.fy);
+    e = launch_gather(p, static_cast<float*>(b->staging), b->stream);
+  }
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("gather launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  LBM_HIP_TRY(hipMemcpy(out, b->staging, bytes, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_batch_total_cells(const lbm_batch* b, long long* cells) {
+  if (!b || !cells) return fail(LBM_EINVAL, "null batch handle or output");
+  *cells = static_cast<long long>(b->total_cells);
+  return LBM_OK;
+}
+
+int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
+                       double* best_mlups) {
+  // Times `steps` direct launches per candidate shape on a copy of the state (restored after);
+  // every shape computes the same bits.
+  if (!b || steps < 1) return fail(LBM_EINVAL, "null batch handle or steps < 1");
+  const size_t bytes = b->total_elems * b->rs();
+  void* backup = nullptr;
+  if (hipMalloc(&backup, bytes) != hipSuccess) return fail(LBM_ENOMEM, "autotune backup allocation failed");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  LBM_HIP_TRY(hipMemcpy(backup, b->current(), bytes, hipMemcpyDeviceToDevice));
+  const int keep_parity = b->parity;
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  cus = std::max(cus, 1);
+  double best = -1.0;
+  int btx = b->tile_x, brows = b->tile_rows, bwg = b->workgroups;
+  for (int tx : {64, 128, 256, 512})
+    for (int r : {1, 2, 4, 8, 16})
+      for (int wgm : {2, 4, 8, 16}) {
+        b->tile_x = tx;
+        b->tile_rows = r;
+        b->workgroups = cus * wgm;
+        LBM_TRY(b->build_tiles());
+        LBM_TRY(b->step_direct());  // warm-up
+        LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        for (int k = 0; k < steps; ++k) LBM_TRY(b->step_direct());
+        LBM_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+        LBM_HIP_TRY(hipEventSynchronize(b->ev1));
+        float ms = 0.0f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        const double mlups = static_cast<double>(b->total_cells) * steps / (ms * 1e-3) / 1e6;
+        if (mlups > best) {
+          best = mlups;
+          btx = tx;
+          brows = r;
+          bwg = cus * wgm;
+        }
+      }
+  b->tile_x = btx;
+  b->tile_rows = brows;
+  b->workgroups = bwg;
+  LBM_TRY(b->build_tiles());
+  b->parity = keep_parity;
+  LBM_HIP_TRY(hipMemcpy(b->current(), backup, bytes, hipMemcpyDeviceToDevice));
+  LBM_HIP_TRY(hipFree(backup));
+  if (tile_x) *tile_x = btx;
+  if (tile_rows) *tile_rows = brows;
+  if (workgroups) *workgroups = bwg;
+  if (best_mlups) *best_mlups = best;
+  return LBM_OK;
+}
+
+}  // extern "C"

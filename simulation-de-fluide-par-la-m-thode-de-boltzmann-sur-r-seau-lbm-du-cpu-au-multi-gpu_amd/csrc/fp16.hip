@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const int nx = p.nx;
   const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+  if (__builtin_amdgcn_readfirstlane(x) >= nx) return;  // whole wave past the last column
   const bool in_x = x < nx;
   const unsigned xc = in_x ? x : nx - 1;
   constexpr unsigned B = sizeof(uint16_t);

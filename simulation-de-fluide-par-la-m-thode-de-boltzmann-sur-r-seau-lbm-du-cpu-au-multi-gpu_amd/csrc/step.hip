@@ -18,7 +18,7 @@
 //
 // Floating point: -ffp-contract=off and IEEE division, so float and double steps are
 // bit-identical to the reference's v0.
-#include "device_common.hpp"
+#include "step_body.hpp"
 #include "kernels.hpp"
 
 namespace lbmhip {
@@ -29,49 +29,12 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
-  const int nx = p.nx;
   const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-  const bool in_x = x < nx;
-  const unsigned xc = in_x ? x : nx - 1;  // lanes past the edge compute on a valid address
-  constexpr unsigned B = sizeof(T);
-  const Cols cols{(xc == 0 ? nx - 1 : xc - 1) * B, xc * B,
-                  (xc + 1 == static_cast<unsigned>(nx) ? 0 : xc + 1) * B};
-  const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;  // wave-uniform 64-column segment
   const int y0 = p.row0 + ty * p.rows_per_tile;
   const int y1 = min(y0 + p.rows_per_tile, p.row0 + p.nrows);
-  Planes g, gn;
-  g.init(p.g, p.N);
-  gn.init(p.gn, p.N);
-  // Force planes (only dereferenced when kField; a null base is never read otherwise).
-  const Rsrc fxr = make_rsrc(p.force), fyr = make_rsrc(p.force + (kField ? p.N : 0));
-
-  for (int y = y0; y < y1; ++y) {
-    int ym = y - 1, yp = y + 1;
-    if (p.wrap_y) {
-      ym = ym < 0 ? p.ny - 1 : ym;
-      yp = yp >= p.ny ? 0 : yp;
-    }
-    const Rows rows{static_cast<unsigned>(ym * nx) * B, static_cast<unsigned>(y * nx) * B,
-                    static_cast<unsigned>(yp * nx) * B};
-    T v[kQ];
-    pull_streamed<T>(g, rows, cols, v);
-    bool active = in_x;
-    if (p.segflags[static_cast<size_t>(y) * p.nseg + seg]) {  // wave-uniform branch
-      const uint32_t link = in_x ? p.links[static_cast<size_t>(y) * nx + xc] : kLinkSolid;
-      active = !(link & kLinkSolid);
-      if (active) apply_links<T, kOpen>(link, g, rows, cols, p.eq_in, v);
-    }
-    if (active) {
-      T fx = p.fx, fy = p.fy;
-      if constexpr (kField) {
-        fx = Stored<T>::load(fxr, cols.c, rows.c);
-        fy = Stored<T>::load(fyr, cols.c, rows.c);
-      }
-      collide(v, fx, fy, p.omega, p.fp);
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) Stored<T>::store(gn.r[i], cols.c, rows.c, v[i]);
-    }
-  }
+  const DomainView<T> d{p.g,  p.gn, p.links, p.segflags, p.force,  p.N,  p.nx,
+                        p.ny, p.nseg, p.wrap_y, p.omega, p.fp,    p.fx, p.fy};
+  update_column<T, kOpen, kField>(d, x, y0, y1, p.eq_in);
 }
 
 // ---- read-back kernels: one thread per cell, gather with the per-cell link code -------------

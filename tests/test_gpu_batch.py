@@ -1,0 +1,76 @@
+"""Batched persistent step: 256 independent domains of uneven sizes and parameters in one launch
+per step, 100-step HIP graphs. Bar: every domain bit-identical to the single-domain step (and,
+for small domains, to the oracle)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+def single(lbm, d, steps, prec="float", init=(1.0, 0.0, 0.0)):
+    s = lbm.Solver(d["nx"], d["ny"], d["tau"], d["fx"], d["fy"], d["solid"], prec)
+    s.init_equilibrium(*init)
+    s.run(steps)
+    out = s.populations()
+    s.close()
+    return out
+
+
+def test_256_domains_100_step_graph_bitwise(lbm):
+    doms = lbm.random_domains(seed=0, n=256)
+    b = lbm.Batch(doms, "float")
+    b.set_graph_steps(100)
+    b.run(100)
+    assert b.total_cells() == sum(d["nx"] * d["ny"] for d in doms)
+    bad = []
+    for k, d in enumerate(doms):
+        if np.count_nonzero(bits(b.populations(k)) != bits(single(lbm, d, 100))):
+            bad.append(k)
+    assert not bad, f"domains differing from the single-domain step: {bad[:10]}"
+
+
+@pytest.mark.parametrize("prec", ("float", "double"))
+def test_odd_graph_length_shapes_and_oracle(lbm, oracle_mod, prec):
+    doms = lbm.random_domains(seed=5, n=12, nx_range=(20, 150), ny_range=(8, 70),
+                              radius_range=(2, 8))
+    init = (1.0, 0.01, -0.005)
+    b = lbm.Batch(doms, prec, init=init)
+    b.set_graph_steps(7)
+    b.run(11)
+    b.set_shape(64, 16, 5)
+    b.run(6)
+    b.set_shape(512, 1, 3)
+    b.set_graph_steps(4)
+    b.run(6)  # 23 steps in total: graphs of both parities and direct launches
+    for k, d in enumerate(doms):
+        ref = single(lbm, d, 23, prec, init)
+        assert np.count_nonzero(bits(b.populations(k)) != bits(ref)) == 0, k
+        if k < 4:
+            o = oracle_mod.Oracle(d["nx"], d["ny"], d["tau"], d["fx"], d["fy"], d["solid"], prec, "v0")
+            o.init_equilibrium(*init)
+            o.run(23)
+            assert np.count_nonzero(bits(o.populations()) != bits(ref)) == 0, k
+
+
+def test_autotune_keeps_results(lbm):
+    doms = lbm.random_domains(seed=9, n=16, nx_range=(32, 300), ny_range=(32, 200))
+    b = lbm.Batch(doms, "float")
+    tx, tr, wg, mlups = b.autotune(steps=2)
+    assert tx % 64 == 0 and tr >= 1 and wg >= 1 and mlups > 0
+    b.run(30)
+    for k in (0, 7, 15):
+        assert np.count_nonzero(bits(b.populations(k)) != bits(single(lbm, doms[k], 30))) == 0
+
+
+def test_batch_rejects_invalid_domains(lbm):
+    d = lbm.random_domains(seed=1, n=1, nx_range=(40, 40), ny_range=(40, 40))[0]
+    bad = dict(d, tau=0.5)
+    with pytest.raises(ValueError, match="tau"):
+        lbm.Batch([d, bad])
