@@ -83,20 +83,130 @@ def cpu_baseline(nx, ny, steps=10, warmup=2):
             "kind": kind, "sample": sample, "seconds": sec}
 
 
+def form_copy_ceiling(lbm, args):
+    nbytes = 2_415_919_104  # one fp32 population buffer of the 8192^2 grid (>> 256 MB MALL)
+    copy_gbs, memcpy_gbs = lbm.copy_ceiling(nbytes, 20)
+    ceiling = max(copy_gbs, memcpy_gbs)
+    return {"copy_kernel_gbs": round(copy_gbs, 1), "memcpy_dtod_gbs": round(memcpy_gbs, 1),
+            "bytes_per_array": nbytes, "reps": 20,
+            "mlups_ceiling_fp32": round(ceiling * 1e3 / 72, 1),
+            "mlups_ceiling_fp16": round(ceiling * 1e3 / 36, 1)}
+
+
+def form_precisions(lbm, args):
+    """fp64 and FP16-storage steps on the same instance; FP16 velocity error vs FP64."""
+    import numpy as np
+
+    nx, ny = args.nx, args.ny
+    solid = lbm.make_channel(nx, ny)
+    fluid = ~solid.astype(bool)
+    out, ux = {}, {}
+    for prec, steps in (("double", 200), ("fp16", 200)):
+        s = lbm.Solver(nx, ny, 0.6, 1e-6, 0.0, solid, prec)
+        s.init_equilibrium(1.0, 0.0, 0.0)
+        m0 = s.total_mass()
+        s.run(20)
+        t = s.timed_run(steps - 20)
+        mlups = nx * ny * (steps - 20) / t / 1e6
+        ux[prec] = s.macroscopic()[1][fluid].astype(np.float64)
+        out[f"plain_{prec}"] = {"mlups": round(mlups, 1), "timed_steps": steps - 20,
+                                "hbm_gbs_algorithmic": round(mlups * 1e6 * BYTES_PER_CELL[prec] / 1e9, 1),
+                                "mass_drift_fp64_reduction": (s.total_mass() - m0) / m0}
+        s.close()
+        log(f"{prec}: {mlups:.0f} MLUPS")
+    d = ux["fp16"] - ux["double"]
+    out["plain_fp16"]["ux_rel_l2_vs_fp64_after_200_steps"] = float(
+        np.sqrt(np.sum(d * d) / np.sum(ux["double"] ** 2)))
+    return out
+
+
+def form_batched(lbm, args):
+    doms = lbm.random_domains(seed=0, n=256)
+    b = lbm.Batch(doms, "float")
+    tx, tr, wg, tuned = b.autotune(steps=3)
+    b.set_graph_steps(100)
+    b.run(100)
+    t = b.timed_run(1000)
+    cells = b.total_cells()
+    mlups = cells * 1000 / t / 1e6
+    b.close()
+    log(f"batched: {mlups:.0f} MLUPS over {cells} cells (tile {tx}x{tr}, {wg} WGs)")
+    return {"domains": 256, "seed": 0, "cells": cells, "mlups": round(mlups, 1),
+            "timed_steps": 1000, "graph_steps": 100, "tile_x": tx, "tile_rows": tr,
+            "workgroups": wg, "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / 1e9, 1)}
+
+
+def form_strip(lbm, args):
+    import numpy as np
+
+    nx, nyl = args.nx, args.ny // 8
+    st = lbm.Strip(nx, nyl, "float")
+    st.set_params(0.6, 1e-6, 0.0)
+    m = np.zeros((nyl + 2, nx), np.uint8)
+    st.upload_solid(m)
+    st.timed_steps(10)
+    t = st.timed_steps(100)
+    mlups = nx * nyl * 100 / t / 1e6
+    st.close()
+    log(f"strip {nx}x({nyl}+2): {mlups:.0f} MLUPS")
+    return {"shape": f"{nx}x({nyl}+2)", "mlups": round(mlups, 1), "timed_steps": 100,
+            "note": "one GPU's share of 8 strips, ghost rows refreshed on device each step"}
+
+
+def form_adjoint(lbm, args):
+    import numpy as np
+
+    nx, ny, T, avg = 512, 128, 2000, 200
+    solid = lbm.add_disc(lbm.make_channel(nx, ny), nx, ny, 128.0, 64.0, 12.0)
+    rng = np.random.default_rng(2024)
+    F = rng.normal(0.0, 1e-7, (2, ny, nx))
+    F[0] += 1e-5
+    out = {}
+    for snaps in (45, T + 1):
+        a = lbm.Adjoint(nx, ny, 0.7, solid, T, (384, 48, 16, 32), avg, snaps, "float")
+        a.set_force_field(F)
+        a.forward()
+        J, tf = a.forward()
+        a.gradient()
+        J2, dF, dtau, tg, fsteps = a.gradient()
+        out[f"snapshots_{snaps}"] = {"forward_ms": round(tf * 1e3, 3), "gradient_ms": round(tg * 1e3, 3),
+                                     "cost_vs_forward": round(tg / tf, 3),
+                                     "recomputed_forward_steps": fsteps, "J": J, "dJ_dtau": dtau}
+        a.close()
+        log(f"adjoint snapshots={snaps}: forward {tf * 1e3:.2f} ms, gradient {tg * 1e3:.2f} ms "
+            f"(x{tg / tf:.2f})")
+    return out
+
+
+def form_cylinder(lbm, args):
+    out = {}
+    for prec in ("float", "double", "fp16"):
+        r = lbm.cylinder_instance(prec)
+        out[prec] = {"st_fft": round(r["st_fft"], 5), "st_zero_crossing": round(r["st_zero"], 5),
+                     "periods": r["periods"], "U_window": round(r["U_window"], 6),
+                     "Re_window": round(r["Re_window"], 2)}
+        log(f"cylinder {prec}: St {r['st_fft']:.4f} / {r['st_zero']:.4f} over {r['periods']} periods, "
+            f"Re {r['Re_window']:.1f}")
+    out["fp16_st_rel_error_vs_fp64"] = abs(out["fp16"]["st_fft"] - out["double"]["st_fft"]) / out["double"]["st_fft"]
+    out["tau"], out["force"] = r["tau"], r["force"]
+    return out
+
+
+FORMS = {"copy_ceiling": form_copy_ceiling, "precisions": form_precisions,
+         "batched": form_batched, "strip": form_strip, "adjoint": form_adjoint,
+         "cylinder": form_cylinder}
+
+
 def forms(lbm, args):
     out = {}
-    t0 = time.time()
-    copy_gbs, memcpy_gbs = lbm.copy_ceiling(1 << 31, 20)
-    ceiling = max(copy_gbs, memcpy_gbs)
-    out["copy_ceiling"] = {"copy_kernel_gbs": copy_gbs, "memcpy_dtod_gbs": memcpy_gbs,
-                           "bytes_per_array": 1 << 31, "mlups_ceiling_fp32": ceiling * 1e3 / 72}
-    log(f"copy ceiling {copy_gbs:.0f} / {memcpy_gbs:.0f} GB/s ({time.time() - t0:.1f} s)")
-    for prec, steps in (("double", 100), ("fp16", 200)):
-        t, info, _ = run_plain(lbm, args.nx, args.ny, steps, 20, prec)
-        mlups = args.nx * args.ny * steps / t / 1e6
-        out[f"plain_{prec}"] = {"mlups": mlups, "steps": steps,
-                                "hbm_gbs": mlups * 1e6 * BYTES_PER_CELL[prec] / 1e9}
-        log(f"{prec}: {mlups:.0f} MLUPS")
+    for name, fn in FORMS.items():
+        t0 = time.time()
+        try:
+            out[name] = fn(lbm, args)
+        except Exception as e:  # one form failing must not lose the headline line
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"form {name} failed: {e}")
+        log(f"form {name}: {time.time() - t0:.1f} s")
     return out
 
 
@@ -110,7 +220,8 @@ def main():
     ap.add_argument("--ny", type=int, default=8192)
     ap.add_argument("--block", type=str, default=None, help="BXxROWS launch shape")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--forms", action="store_true", help="also time the other forms")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other forms")
+    ap.add_argument("--forms", action="store_true", help="(default) time the other forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: every box here has a single MI355X and nothing in "
@@ -151,10 +262,11 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile()},
     }
-    if args.forms:
+    if not args.no_forms:
         line["forms"] = forms(lbm, args)
-        ceil = line["forms"]["copy_ceiling"]["mlups_ceiling_fp32"]
-        line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / ceil, 4)
+        ceil = line["forms"].get("copy_ceiling", {}).get("mlups_ceiling_fp32")
+        if ceil:
+            line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / ceil, 4)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(args.nx, args.ny)

@@ -120,6 +120,11 @@ int lbm_set_fp16_scale(lbm_solver* s, double scale);
  * fastest on the handle. The state is saved and restored around the trials. */
 int lbm_autotune(lbm_solver* s, int steps, int* best_block_x, int* best_rows, double* best_mlups);
 
+/* No reference counterpart (the reference samples a velocity probe, driver.hpp:186-251):
+ * momentum-exchange force on the solids bordering the fluid cells of the box [x0, x0 + w) x
+ * [y0, y0 + h), FP64, deterministic; drag = *fx, lift = *fy (lattice units per step). */
+int lbm_momentum_exchange(lbm_solver* s, int x0, int y0, int w, int h, double* fx, double* fy);
+
 /* Reference poiseuille_force(H, tau, u_max) in the solver precision (geometry.hpp:24-26). */
 double lbm_poiseuille_force(int precision, int H, double tau, double u_max);
 
@@ -192,6 +197,10 @@ int lbm_adjoint_forward(lbm_adjoint* a, double* J, double* seconds);
 /* J and its gradient: dJ/dF (2N reals, planar) and dJ/dtau. */
 int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_dtau,
                          double* seconds, long long* forward_steps);
+
+/* Host only (no device work): forward steps the checkpointing schedule recomputes to reverse
+ * `states` states with `slots` stored states (slot 0 holds the initial state). */
+int lbm_checkpoint_forward_steps(int states, int slots, long long* steps);
 
 /* ---- HBM copy ceiling (the counterpart of bench/gpu_bandwidth.cu) ---------------------------- */
 /* Best-of-reps copy bandwidth (GB/s, 2 * bytes / time) of a grid-stride copy kernel and of

@@ -89,6 +89,7 @@ def _declare(L):
         "lbm_set_fp16_scale": ([vp, d], i),
         "lbm_autotune": ([vp, i, ip, ip, dp], i),
         "lbm_poiseuille_force": ([i, i, d, d], d),
+        "lbm_momentum_exchange": ([vp, i, i, i, i, dp, dp], i),
         "lbm_copy_ceiling": ([sz, i, dp, dp], i),
         "lbm_strip_create": ([i, i, i, C.POINTER(vp)], i),
         "lbm_strip_destroy": ([vp], i),
@@ -117,6 +118,7 @@ def _declare(L):
         "lbm_adjoint_set_tau": ([vp, d], i),
         "lbm_adjoint_forward": ([vp, dp, dp], i),
         "lbm_adjoint_gradient": ([vp, dp, vp, dp, dp, llp], i),
+        "lbm_checkpoint_forward_steps": ([i, i, llp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -288,6 +290,13 @@ class Solver:
         out = np.empty(9 * self.N, dtype=self.dtype)
         check(lib().lbm_populations(self._h, out.ctypes.data))
         return out
+
+    def momentum_exchange(self, x0, y0, w, h) -> tuple[float, float]:
+        """(drag, lift) on the solids bordering the fluid cells of a box (FP64)."""
+        fx, fy = C.c_double(), C.c_double()
+        check(lib().lbm_momentum_exchange(self._h, int(x0), int(y0), int(w), int(h), C.byref(fx),
+                                          C.byref(fy)))
+        return fx.value, fy.value
 
     def populations_at(self, x, y) -> np.ndarray:
         out = np.empty(9, dtype=self.dtype)
@@ -533,6 +542,13 @@ class Batch:
 
 # ---- discrete adjoint ------------------------------------------------------------------------
 
+def checkpoint_forward_steps(states: int, slots: int) -> int:
+    """Forward steps recomputed by the checkpointing schedule (host computation only)."""
+    n = C.c_longlong()
+    check(lib().lbm_checkpoint_forward_steps(int(states), int(slots), C.byref(n)))
+    return n.value
+
+
 class Adjoint:
     """Gradient of J = mean u_x over a probe window, averaged over the last `avg_steps` states of
     a `steps`-step run from rest, w.r.t. a per-cell force field and tau (discrete adjoint with
@@ -587,6 +603,60 @@ class Adjoint:
 
 
 # ---- analysis (scripts/strouhal.py:57-81 method) ---------------------------------------------
+
+def equilibrium_field(ux: np.ndarray, uy: np.ndarray, rho: np.ndarray | float = 1.0,
+                      dtype=np.float64) -> np.ndarray:
+    """Shifted equilibrium g_eq = w (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)) of a velocity
+    field (src/common/d2q9.hpp:40-42), as a post-streaming state in the v0 AoS layout."""
+    cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.float64)
+    cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.float64)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    u = np.asarray(ux, np.float64).reshape(-1, 1)
+    v = np.asarray(uy, np.float64).reshape(-1, 1)
+    r = np.broadcast_to(np.asarray(rho, np.float64).reshape(-1, 1), u.shape)
+    cu = u * cx + v * cy
+    g = w * ((r - 1.0) + r * (3 * cu + 4.5 * cu * cu - 1.5 * (u * u + v * v)))
+    return g.astype(dtype).reshape(-1)
+
+
+def cylinder_instance(precision="float", nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0,
+                      Re=100.0, U=0.05, steps=40000, sample_every=20, window=0.5):
+    """Force-driven walled channel (periodic in x) with a staircase cylinder: tau and the body
+    force are set for Re = U D / nu with a Poiseuille profile of mean velocity U (peak 1.5 U),
+    which is also the initial state. The lift (momentum exchange on the cylinder) is sampled
+    every `sample_every` steps; the Strouhal number f D / U_w uses the mean fluid velocity U_w
+    over the analysis window (the last `window` fraction of the run).
+    Returns a dict with St (FFT and zero crossings), periods, U_w, Re_w, tau, force and the
+    lift signal."""
+    nu = U * D / Re
+    tau = 0.5 + 3.0 * nu
+    H = ny - 2
+    umax = 1.5 * U
+    force = 8.0 * nu * umax / (H * H)
+    solid = np.maximum(make_channel(nx, ny), make_cylinder(nx, ny, xc, yc, D))
+    yy = np.arange(ny, dtype=np.float64) - 0.5  # distance from the lower halfway wall
+    prof = np.where((yy > 0) & (yy < H), 4 * umax * yy * (H - yy) / (H * H), 0.0)
+    ux = np.repeat(prof[:, None], nx, axis=1).reshape(-1)
+    S = equilibrium_field(ux, np.zeros_like(ux), 1.0, real_dtype(precision))
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    s = Solver(nx, ny, tau, force, 0.0, solid, precision)
+    s.set_populations(S)
+    r = int(np.ceil(D / 2)) + 2
+    box = (int(xc) - r, int(yc) - r, 2 * r + 1, 2 * r + 1)
+    fluid = float(nx * ny - int(solid.sum()))
+    lift, umean = [], []
+    for _ in range(steps // sample_every):
+        s.run(sample_every)
+        lift.append(s.momentum_exchange(*box)[1])
+        umean.append(s.momentum()[0] / fluid)
+    lift, umean = np.array(lift), np.array(umean)
+    k0 = int(len(lift) * (1 - window))
+    Uw = float(umean[k0:].mean())
+    st_fft, st_zero, periods = strouhal(lift[k0:], float(sample_every), D, Uw)
+    s.close()
+    return dict(st_fft=st_fft, st_zero=st_zero, periods=periods, U_window=Uw,
+                Re_window=Uw * D / nu, tau=tau, force=force, lift=lift, umean=umean)
+
 
 def strouhal(uy: np.ndarray, dt: float, D: float, U: float) -> tuple[float, float, int]:
     """Shedding Strouhal number of a transverse-velocity probe signal sampled every dt steps.

@@ -66,6 +66,10 @@ struct Revolve {
     if (b >= a - 1) return 1;  // enough slots: store every state
     return split[static_cast<size_t>(a) * (s + 1) + std::min(b, s)];
   }
+  int64_t forward_steps(int a, int b) const {
+    if (b >= a - 1) return a - 1;
+    return cost[static_cast<size_t>(a) * (s + 1) + std::min(b, s)];
+  }
 };
 
 }  // namespace
@@ -352,6 +356,16 @@ int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int ste
     return bail(fail(LBM_EHIP, "adjoint upload failed"));
   a->rv.build(a->T + 1, a->snapshots - 1);
   *out = a;
+  return LBM_OK;
+}
+
+int lbm_checkpoint_forward_steps(int states, int slots, long long* steps) {
+  if (states < 1 || slots < 1 || !steps) return fail(LBM_EINVAL, "need states >= 1, slots >= 1");
+  if (slots - 1 < states - 1 && std::min(slots - 1, Revolve::kMaxTableSlots) * 1.0 * states * states > 4e9)
+    return fail(LBM_EINVAL, "schedule table too large");
+  Revolve r;
+  r.build(states, slots - 1);
+  *steps = r.forward_steps(states, slots - 1);
   return LBM_OK;
 }
 

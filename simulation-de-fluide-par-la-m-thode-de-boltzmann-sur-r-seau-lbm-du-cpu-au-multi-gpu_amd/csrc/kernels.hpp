@@ -108,6 +108,12 @@ hipError_t launch_reduce(const GatherParams<T>& p, double* partials, double* out
 hipError_t launch_reduce_fp16(const GatherParamsH& p, double* partials, double* out,
                               hipStream_t stream);
 
+// Momentum-exchange force (FP64, out[0..1]) on the solids next to the fluid cells of the box
+// [x0, x0 + w) x [y0, y0 + h) (diag.hip). S = float, double or uint16_t (FP16 storage).
+template <class S>
+hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,
+                           int w, int h, float inv_scale, double* out, hipStream_t stream);
+
 // Grid-stride copy (HBM ceiling), n floats.
 hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block,
                        hipStream_t stream);

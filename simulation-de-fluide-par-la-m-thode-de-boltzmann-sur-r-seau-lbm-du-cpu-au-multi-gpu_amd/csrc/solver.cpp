@@ -244,6 +244,11 @@ int lbm_autotune(lbm_solver* s, int steps, int* best_block_x, int* best_rows, do
   return s->autotune(steps, best_block_x, best_rows, best_mlups);
 }
 
+int lbm_momentum_exchange(lbm_solver* s, int x0, int y0, int w, int h, double* fx, double* fy) {
+  LBM_TRY(check_handle(s));
+  return s->momentum_exchange(x0, y0, w, h, fx, fy);
+}
+
 double lbm_poiseuille_force(int precision, int H, double tau, double u_max) {
   // real(8) * viscosity(tau) * u_max / (real(H) * real(H)), viscosity = (tau - 1/2) / 3,
   // all in the solver's real (src/common/geometry.hpp:24-26, d2q9.hpp:30).

@@ -372,6 +372,29 @@ int SolverState::reduce(double (&out)[4]) {
   return LBM_OK;
 }
 
+int SolverState::momentum_exchange(int x0, int y0, int w, int h, double* fx, double* fy) {
+  if (x0 < 0 || y0 < 0 || w < 1 || h < 1 || x0 + w > nx || y0 + h > ny)
+    return fail(LBM_EINVAL, "momentum-exchange box outside the grid");
+  double* out = red + 4 * kReduceBlocks;
+  hipError_t e;
+  if (prec == LBM_FLOAT)
+    e = launch_exchange(static_cast<const float*>(g), links, cells(), nx, x0, y0, w, h, 1.0f, out,
+                        stream);
+  else if (prec == LBM_DOUBLE)
+    e = launch_exchange(static_cast<const double*>(g), links, cells(), nx, x0, y0, w, h, 1.0f, out,
+                        stream);
+  else
+    e = launch_exchange(static_cast<const uint16_t*>(g), links, cells(), nx, x0, y0, w, h,
+                        static_cast<float>(1.0 / fp16_scale), out, stream);
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("exchange launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  double r[2];
+  LBM_HIP_TRY(hipMemcpy(r, out, sizeof r, hipMemcpyDeviceToHost));
+  if (fx) *fx = r[0];
+  if (fy) *fy = r[1];
+  return LBM_OK;
+}
+
 int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_mlups) {
   // Trials run on a copy of the state, which is restored afterwards. Every launch shape
   // computes the same bits (each cell's arithmetic does not depend on the shape).

@@ -47,6 +47,7 @@ struct SolverState {
   int macroscopic(void* rho, void* ux, void* uy);
   int reduce(double (&out)[4]);
   int autotune(int steps, int* bx, int* rows, double* mlups);
+  int momentum_exchange(int x0, int y0, int w, int h, double* fx, double* fy);
 
  private:
   int upload_links();
