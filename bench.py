@@ -199,7 +199,9 @@ FORMS = {"copy_ceiling": form_copy_ceiling, "precisions": form_precisions,
 
 def forms(lbm, args):
     out = {}
-    for name, fn in FORMS.items():
+    wanted = list(FORMS) if args.forms in ("all", None) else args.forms.split(",")
+    for name in wanted:
+        fn = FORMS[name]
         t0 = time.time()
         try:
             out[name] = fn(lbm, args)
@@ -221,7 +223,10 @@ def main():
     ap.add_argument("--block", type=str, default=None, help="BXxROWS launch shape")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms")
-    ap.add_argument("--forms", action="store_true", help="(default) time the other forms")
+    ap.add_argument("--forms", default="all",
+                    help="comma-separated subset of: " + ",".join(FORMS) + " (default all)")
+    ap.add_argument("--cpu-steps", type=int, default=400,
+                    help="timed steps of the CPU-baseline sample (8192^2 cells each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: every box here has a single MI355X and nothing in "
@@ -269,7 +274,7 @@ def main():
             line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / ceil, 4)
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(args.nx, args.ny)
+        line["cpu_baseline"] = cpu_baseline(args.nx, args.ny, steps=args.cpu_steps)
         log(f"cpu baseline: {line['cpu_baseline']['value']:.1f} MLUPS on "
             f"{line['cpu_baseline']['cores']} threads ({line['cpu_baseline']['kind']})")
     print(json.dumps(line), flush=True)

@@ -115,7 +115,8 @@ hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, 
                            int w, int h, float inv_scale, double* out, hipStream_t stream);
 
 // Grid-stride copy (HBM ceiling), n floats.
-hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block,
+// variant 0: scalar (the reference's kernel), 1: 16-byte accesses, 2: 16-byte non-temporal.
+hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block, int variant,
                        hipStream_t stream);
 
 }  // namespace lbmhip

@@ -471,8 +471,14 @@ int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
     }
     return best;
   };
-  const double kms = best_ms([&] { (void)launch_copy(a, b, n, cus * 32, 256, st); });
-  LBM_HIP_TRY(hipGetLastError());
+  // The reference's scalar kernel (grid = CUs x 32, block 256) and 16-byte variants over a few
+  // grid sizes; the ceiling is the best of all (a higher ceiling is the conservative choice).
+  double kms = 1e30;
+  for (int variant : {0, 1, 2})
+    for (int per_cu : {8, 32, 128}) {
+      kms = std::min(kms, best_ms([&] { (void)launch_copy(a, b, n, cus * per_cu, 256, variant, st); }));
+      LBM_HIP_TRY(hipGetLastError());
+    }
   const double mms = best_ms([&] {
     (void)hipMemcpyAsync(b, a, n * sizeof(float), hipMemcpyDeviceToDevice, st);
   });

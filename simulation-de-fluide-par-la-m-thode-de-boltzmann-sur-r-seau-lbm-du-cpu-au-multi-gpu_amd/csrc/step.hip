@@ -104,10 +104,22 @@ __global__ void __launch_bounds__(kReduceThreads) reduce_kernel(const GatherPara
     for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = s[k];
 }
 
+// Grid-stride copy, the counterpart of the reference's ceiling kernel
+// (bench/gpu_bandwidth.cu:17-22); kVec floats per access (1 = the reference's form, 4 = one
+// 16-byte access per lane), optionally with non-temporal hints.
+template <int kVec, bool kNt>
 __global__ void copy_kernel(const float* __restrict__ a, float* __restrict__ b, size_t n) {
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x)
-    b[i] = a[i];
+  using V = float __attribute__((ext_vector_type(kVec)));
+  const V* av = reinterpret_cast<const V*>(a);
+  V* bv = reinterpret_cast<V*>(b);
+  const size_t nv = n / kVec;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nv;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    if constexpr (kNt)
+      __builtin_nontemporal_store(__builtin_nontemporal_load(av + i), bv + i);
+    else
+      bv[i] = av[i];
+  }
 }
 
 }  // namespace
@@ -186,9 +198,13 @@ template hipError_t launch_reduce<float>(const GatherParams<float>&, double*, do
 template hipError_t launch_reduce<double>(const GatherParams<double>&, double*, double*,
                                           hipStream_t);
 
-hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block,
+hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block, int variant,
                        hipStream_t stream) {
-  copy_kernel<<<grid, block, 0, stream>>>(a, b, n);
+  switch (variant) {
+    case 0: copy_kernel<1, false><<<grid, block, 0, stream>>>(a, b, n); break;
+    case 1: copy_kernel<4, false><<<grid, block, 0, stream>>>(a, b, n); break;
+    default: copy_kernel<4, true><<<grid, block, 0, stream>>>(a, b, n); break;
+  }
   return hipGetLastError();
 }
 
