@@ -183,10 +183,12 @@ def form_cylinder(lbm, args):
     for prec in ("float", "double", "fp16"):
         r = lbm.cylinder_instance(prec)
         out[prec] = {"st_fft": round(r["st_fft"], 5), "st_zero_crossing": round(r["st_zero"], 5),
-                     "periods": r["periods"], "U_window": round(r["U_window"], 6),
-                     "Re_window": round(r["Re_window"], 2)}
-        log(f"cylinder {prec}: St {r['st_fft']:.4f} / {r['st_zero']:.4f} over {r['periods']} periods, "
+                     "periods": r["periods"], "st_lift_fft": round(r["st_lift_fft"], 5),
+                     "U_window": round(r["U_window"], 6), "Re_window": round(r["Re_window"], 2)}
+        log(f"cylinder {prec}: probe St {r['st_fft']:.4f} (FFT) / {r['st_zero']:.4f} (zero "
+            f"crossings, {r['periods']} periods), lift St {r['st_lift_fft']:.4f}, "
             f"Re {r['Re_window']:.1f}")
+    out["signal"] = "u_y at the wake probe (xc + 3D, yc); St = f D / U_window"
     out["fp16_st_rel_error_vs_fp64"] = abs(out["fp16"]["st_fft"] - out["double"]["st_fft"]) / out["double"]["st_fft"]
     out["tau"], out["force"] = r["tau"], r["force"]
     return out

@@ -22,9 +22,10 @@ namespace lbmhip {
 namespace {
 
 template <class T>
-__global__ void __launch_bounds__(256) adjoint_step_kernel(const AdjointParams<T> p) {
+__global__ void __launch_bounds__(kReduceThreads) adjoint_step_kernel(const AdjointParams<T> p) {
   const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   T dtau = T(0);
+  double jpart = 0.0;  // k * ux at probe cells of a state in the window: J, summed per block
   if (c < p.N) {
     const int nx = p.nx, ny = p.ny;
     const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
@@ -92,8 +93,10 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const AdjointParams<T
         dom += l * (geq - g[i] - T(0.5) * src);
       }
       // direct term of J: k * ux at probe cells, states inside the averaging window
-      if (p.direct && x >= p.px0 && x < p.px0 + p.pw && y >= p.py0 && y < p.py0 + p.ph)
+      if (p.direct && x >= p.px0 && x < p.px0 + p.pw && y >= p.py0 && y < p.py0 + p.ph) {
         A_ux += p.k;
+        jpart = p.kd * static_cast<double>(ux);
+      }
       const T ir = T(1) / rho;
       const T A_mx = A_ux * ir, A_my = A_uy * ir;
       const T A_rho = A_drho - (A_ux * ux + A_uy * uy) * ir;
@@ -108,17 +111,12 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(const AdjointParams<T
       dtau = -(om * om) * dom;  // d omega / d tau = -omega^2
     }
   }
-  // per-block partial of dJ/dtau, accumulated in FP64 into this block's own slot
-  double s[4] = {static_cast<double>(dtau), 0.0, 0.0, 0.0};
-  __shared__ double part[4][4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  s[0] = wave_sum(s[0]);
-  if (lane == 0) part[wave][0] = s[0];
-  __syncthreads();
+  // per-block partials of dJ/dtau and J, accumulated in FP64 into this block's own slots
+  double s[4] = {static_cast<double>(dtau), jpart, 0.0, 0.0};
+  block_sum4(s);
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) t += part[w][0];
-    p.dtau_partials[blockIdx.x] += t;
+    p.partials[2 * blockIdx.x] += s[0];
+    p.partials[2 * blockIdx.x + 1] += s[1];
   }
 }
 

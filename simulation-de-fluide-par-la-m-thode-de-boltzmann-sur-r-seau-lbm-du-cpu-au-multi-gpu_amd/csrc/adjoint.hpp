@@ -18,13 +18,14 @@ struct AdjointParams {
   const uint16_t* links;
   const T* force;      // per-cell force field (fx plane, fy plane)
   T* dforce;           // dJ/dF accumulators (2N)
-  double* dtau_partials;  // one slot per block
+  double* partials;    // two slots per block: dJ/dtau, J
   size_t N;
   int nx, ny;
   T omega, fp;
   int direct;          // 1 if state t lies in the averaging window
   int px0, py0, pw, ph;
   T k;                 // 1 / (window steps * probe cells)
+  double kd;           // the same in FP64, for the J partials
 };
 
 inline unsigned adjoint_blocks(size_t N) { return static_cast<unsigned>((N + 255) / 256); }

@@ -87,7 +87,7 @@ struct lbm_adjoint {
   void* slots = nullptr;
   void* work[2] = {nullptr, nullptr};
   void* mu[2] = {nullptr, nullptr};
-  double* dtau_partials = nullptr;
+  double* partials = nullptr;  // per reverse-kernel block: dJ/dtau, J
   double* jacc = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -112,7 +112,7 @@ struct lbm_adjoint {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (void* p : {static_cast<void*>(links), static_cast<void*>(segflags), force, dforce, slots,
-                    work[0], work[1], mu[0], mu[1], static_cast<void*>(dtau_partials),
+                    work[0], work[1], mu[0], mu[1], static_cast<void*>(partials),
                     static_cast<void*>(jacc)})
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -137,6 +137,7 @@ struct lbm_adjoint {
     p.tiles_x = (nx + 63) / 64;
     p.nseg = nseg;
     p.wrap_y = 1;
+    p.one_rsrc = 1;
     const T t = static_cast<T>(tau);
     p.omega = T(1) / t;
     p.fp = T(1) - T(0.5) * p.omega;
@@ -175,7 +176,7 @@ struct lbm_adjoint {
     p.links = links;
     p.force = static_cast<const T*>(force);
     p.dforce = static_cast<T*>(dforce);
-    p.dtau_partials = dtau_partials;
+    p.partials = partials;
     p.N = N();
     p.nx = nx;
     p.ny = ny;
@@ -187,12 +188,13 @@ struct lbm_adjoint {
     p.py0 = py0;
     p.pw = pw;
     p.ph = ph;
-    p.k = static_cast<T>(1.0 / (static_cast<double>(avg) * pw * ph));
+    p.kd = 1.0 / (static_cast<double>(avg) * pw * ph);
+    p.k = static_cast<T>(p.kd);
     const hipError_t e = launch_adjoint_step(p, stream);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("adjoint step: ") + hipGetErrorString(e));
     mu_cur ^= 1;
     have_mu = true;
-    if (p.direct) LBM_TRY(probe<T>(R));  // J falls out of the reverse sweep as well
+
     return LBM_OK;
   }
   int reverse_any(int t, const void* R) {
@@ -237,7 +239,7 @@ struct lbm_adjoint {
   template <class T>
   int fills() {
     const hipError_t e1 = launch_fill(static_cast<T*>(dforce), 2 * N(), T(0), stream);
-    const hipError_t e2 = launch_fill(dtau_partials, static_cast<size_t>(adjoint_blocks(N())), 0.0, stream);
+    const hipError_t e2 = launch_fill(partials, 2 * static_cast<size_t>(adjoint_blocks(N())), 0.0, stream);
     const hipError_t e3 = launch_fill(jacc, 1, 0.0, stream);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(LBM_EHIP, "fill launch failed");
     return LBM_OK;
@@ -345,7 +347,7 @@ int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int ste
       hipMalloc(&a->slots, sb * a->snapshots) != hipSuccess || hipMalloc(&a->work[0], sb) != hipSuccess ||
       hipMalloc(&a->work[1], sb) != hipSuccess || hipMalloc(&a->mu[0], sb) != hipSuccess ||
       hipMalloc(&a->mu[1], sb) != hipSuccess ||
-      hipMalloc(&a->dtau_partials, sizeof(double) * adjoint_blocks(a->N())) != hipSuccess ||
+      hipMalloc(&a->partials, 2 * sizeof(double) * adjoint_blocks(a->N())) != hipSuccess ||
       hipMalloc(&a->jacc, sizeof(double)) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "device allocation of the adjoint failed"));
   const Links L = build_links(nx, ny, a->solid.data(), false, true);
@@ -406,17 +408,17 @@ int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_
   if (!a) return fail(LBM_EINVAL, "null adjoint handle");
   if (!a->grad_exec) LBM_TRY(a->build_gradient_graph());
   LBM_TRY(a->run_graph(a->grad_exec, seconds));
-  if (J) LBM_HIP_TRY(hipMemcpy(J, a->jacc, sizeof(double), hipMemcpyDeviceToHost));
   if (dJ_dforce)
     LBM_HIP_TRY(hipMemcpy(dJ_dforce, a->dforce, 2 * a->N() * a->rs(), hipMemcpyDeviceToHost));
-  if (dJ_dtau) {
-    std::vector<double> part(adjoint_blocks(a->N()));
-    LBM_HIP_TRY(hipMemcpy(part.data(), a->dtau_partials, part.size() * sizeof(double),
-                          hipMemcpyDeviceToHost));
-    double s = 0.0;
-    for (double v : part) s += v;
-    *dJ_dtau = s;
+  std::vector<double> part(2 * static_cast<size_t>(adjoint_blocks(a->N())));
+  LBM_HIP_TRY(hipMemcpy(part.data(), a->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double st = 0.0, sj = 0.0;  // fixed order: deterministic
+  for (size_t b = 0; b < part.size() / 2; ++b) {
+    st += part[2 * b];
+    sj += part[2 * b + 1];
   }
+  if (dJ_dtau) *dJ_dtau = st;
+  if (J) *J = sj;
   if (forward_steps) *forward_steps = a->forward_steps_scheduled;
   return LBM_OK;
 }

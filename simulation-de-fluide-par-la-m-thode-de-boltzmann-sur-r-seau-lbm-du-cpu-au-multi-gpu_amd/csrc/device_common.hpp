@@ -78,13 +78,48 @@ struct Stored<uint16_t> {
   }
 };
 
-// Nine plane resources of one population buffer.
+// Nine plane resources of one population buffer (any size: each plane < 4 GiB).
 struct Planes {
   Rsrc r[kQ];
   template <class S>
   __device__ __forceinline__ void init(const S* base, size_t N) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) r[i] = make_rsrc(base + i * N);
+  }
+  template <class S>
+  __device__ __forceinline__ typename Stored<S>::C load(int i, unsigned vo, unsigned so) const {
+    return Stored<S>::load(r[i], vo, so);
+  }
+  template <class S, int kAux>
+  __device__ __forceinline__ void store(int i, unsigned vo, unsigned so, typename Stored<S>::C v) const {
+    Stored<S>::template store<kAux>(r[i], vo, so, v);
+  }
+  __device__ __forceinline__ void store_bits(int i, unsigned vo, unsigned so, unsigned short h) const {
+    Stored<uint16_t>::store_bits(r[i], vo, so, h);
+  }
+};
+
+// One resource for all nine planes, plane i at byte offset i * stride folded into the scalar
+// offset: 4 SGPRs instead of 36 per buffer (higher occupancy: the plane resources otherwise
+// make the step SGPR-limited). Valid when the nine planes span less than 4 GiB.
+struct PlanesOne {
+  Rsrc r;
+  unsigned stride;
+  template <class S>
+  __device__ __forceinline__ void init(const S* base, size_t N) {
+    r = make_rsrc(base);
+    stride = static_cast<unsigned>(N * sizeof(S));
+  }
+  template <class S>
+  __device__ __forceinline__ typename Stored<S>::C load(int i, unsigned vo, unsigned so) const {
+    return Stored<S>::load(r, vo, so + static_cast<unsigned>(i) * stride);
+  }
+  template <class S, int kAux>
+  __device__ __forceinline__ void store(int i, unsigned vo, unsigned so, typename Stored<S>::C v) const {
+    Stored<S>::template store<kAux>(r, vo, so + static_cast<unsigned>(i) * stride, v);
+  }
+  __device__ __forceinline__ void store_bits(int i, unsigned vo, unsigned so, unsigned short h) const {
+    Stored<uint16_t>::store_bits(r, vo, so + static_cast<unsigned>(i) * stride, h);
   }
 };
 
@@ -99,22 +134,22 @@ struct Cols {
 };
 
 // The nine pulled populations of a cell: direction i comes from (x - cx_i, y - cy_i).
-template <class S, class C = typename Stored<S>::C>
-__device__ __forceinline__ void pull_streamed(const Planes& g, const Rows& rows, const Cols& cols,
+template <class S, class PS = Planes, class C = typename Stored<S>::C>
+__device__ __forceinline__ void pull_streamed(const PS& g, const Rows& rows, const Cols& cols,
                                               C (&v)[kQ]) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) v[i] = Stored<S>::load(g.r[i], cols.of(kCx[i]), rows.of(kCy[i]));
+  for (int i = 0; i < kQ; ++i) v[i] = g.template load<S>(i, cols.of(kCx[i]), rows.of(kCy[i]));
 }
 
 // Dirty-segment corrections for one cell, from its link code: bounce-back from the cell's own
 // opposite slot (src/v3_cuda_opt/solver_v3.cu:74-75), and the open-x inlet / outlet rules
 // (solver_v3.cu:66-73, src/common/boundary.hpp:53-70).
-template <class S, bool kOpen, class C = typename Stored<S>::C>
-__device__ __forceinline__ void apply_links(uint32_t link, const Planes& g, const Rows& rows,
+template <class S, bool kOpen, class PS = Planes, class C = typename Stored<S>::C>
+__device__ __forceinline__ void apply_links(uint32_t link, const PS& g, const Rows& rows,
                                             const Cols& cols, const C* eq_in, C (&v)[kQ]) {
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
-    if (link & (1u << i)) v[i] = Stored<S>::load(g.r[kOpp[i]], cols.c, rows.c);
+    if (link & (1u << i)) v[i] = g.template load<S>(kOpp[i], cols.c, rows.c);
   if constexpr (kOpen) {
     if (link & kLinkInlet) {
       v[1] = eq_in[1];
@@ -127,8 +162,8 @@ __device__ __forceinline__ void apply_links(uint32_t link, const Planes& g, cons
       const unsigned left = cols.c - static_cast<unsigned>(sizeof(S));
 #pragma unroll
       for (int i : {3, 6, 7})
-        v[i] = (link & outlet_bb_bit(i)) ? Stored<S>::load(g.r[kOpp[i]], left, rows.c)
-                                         : Stored<S>::load(g.r[i], cols.c, rows.of(kCy[i]));
+        v[i] = (link & outlet_bb_bit(i)) ? g.template load<S>(kOpp[i], left, rows.c)
+                                         : g.template load<S>(i, cols.c, rows.of(kCy[i]));
     }
   }
 }

@@ -36,6 +36,8 @@ struct StepParams {
   int wrap_y;          // 1: periodic in y; 0: strip (rows row0-1 and row0+nrows exist)
   T omega, fp, fx, fy; // 1/tau, 1 - omega/2 (host-computed in T, solver_v3.cu:252-253)
   T eq_in[9];          // inlet equilibrium (open x)
+  int store_nt;        // 1: non-temporal stores (periodic, uniform-force kernel only)
+  int one_rsrc;        // 1: one buffer resource for the nine planes when they fit in 4 GiB
 };
 
 // FP16-storage step: halves hold g * scale; arithmetic in fp32 (fp16.hip).

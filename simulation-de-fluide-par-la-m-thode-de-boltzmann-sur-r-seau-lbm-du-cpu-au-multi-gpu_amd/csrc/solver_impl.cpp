@@ -45,6 +45,8 @@ StepParams<T> make_step_params(const SolverState& s) {
   T eq[kQ] = {};
   if (s.open_x) equilibrium_state<T>(T(1), static_cast<T>(s.u_in), T(0), eq);
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i];
+  p.store_nt = s.store_nt;
+  p.one_rsrc = s.one_rsrc;
   return p;
 }
 
@@ -406,11 +408,15 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
   LBM_HIP_TRY(hipMemcpy(backup, g, bytes, hipMemcpyDeviceToDevice));
   const int keep_bx = block_x, keep_rows = rows;
   double best = -1.0;
-  int bbx = keep_bx, brows = keep_rows;
+  int bbx = keep_bx, brows = keep_rows, bnt = store_nt, bone = one_rsrc;
+  for (int one : {1, 0})
+  for (int nt : {0, 1})
   for (int bx : {64, 128, 256, 512, 1024})
     for (int r : {1, 2, 4, 8, 16, 32}) {
       block_x = bx;
       rows = r;
+      store_nt = nt;
+      one_rsrc = one;
       int rc = step();  // warm-up (also checks the launch)
       if (rc != LBM_OK) continue;
       LBM_HIP_TRY(hipEventRecord(ev0, stream));
@@ -424,10 +430,14 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
         best = mlups;
         bbx = bx;
         brows = r;
+        bnt = nt;
+        bone = one;
       }
     }
   block_x = bbx;
   rows = brows;
+  store_nt = bnt;
+  one_rsrc = bone;
   LBM_HIP_TRY(hipMemcpy(g, backup, bytes, hipMemcpyDeviceToDevice));
   LBM_HIP_TRY(hipMemcpy(gn, backup, bytes, hipMemcpyDeviceToDevice));
   LBM_HIP_TRY(hipFree(backup));

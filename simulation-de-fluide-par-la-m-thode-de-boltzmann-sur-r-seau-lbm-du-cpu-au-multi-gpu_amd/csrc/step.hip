@@ -24,7 +24,7 @@
 namespace lbmhip {
 namespace {
 
-template <class T, bool kOpen, bool kField>
+template <class T, bool kOpen, bool kField, int kStoreAux, bool kOne = false>
 __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
@@ -34,7 +34,8 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
   const int y1 = min(y0 + p.rows_per_tile, p.row0 + p.nrows);
   const DomainView<T> d{p.g,  p.gn, p.links, p.segflags, p.force,  p.N,  p.nx,
                         p.ny, p.nseg, p.wrap_y, p.omega, p.fp,    p.fx, p.fy};
-  update_column<T, kOpen, kField>(d, x, y0, y1, p.eq_in);
+  using PS = typename std::conditional<kOne, PlanesOne, Planes>::type;
+  update_column<T, kOpen, kField, kStoreAux, PS>(d, x, y0, y1, p.eq_in);
 }
 
 // ---- read-back kernels: one thread per cell, gather with the per-cell link code -------------
@@ -144,14 +145,24 @@ hipError_t launch_step(const StepParams<T>& p, bool open_x, int block_x, hipStre
   const bool field = p.force != nullptr;
   if (open_x) {
     if (field)
-      step_kernel<T, true, true><<<grid, block_x, 0, stream>>>(p);
+      step_kernel<T, true, true, kAuxNone><<<grid, block_x, 0, stream>>>(p);
     else
-      step_kernel<T, true, false><<<grid, block_x, 0, stream>>>(p);
+      step_kernel<T, true, false, kAuxNone><<<grid, block_x, 0, stream>>>(p);
+  } else if (field) {
+    step_kernel<T, false, true, kAuxNone><<<grid, block_x, 0, stream>>>(p);
   } else {
-    if (field)
-      step_kernel<T, false, true><<<grid, block_x, 0, stream>>>(p);
-    else
-      step_kernel<T, false, false><<<grid, block_x, 0, stream>>>(p);
+    // the hot path: one resource for all planes when they span < 4 GiB (fewer SGPRs)
+    const bool one = 9 * p.N * sizeof(T) < (size_t(1) << 32) && p.one_rsrc;
+    if (p.store_nt) {
+      if (one)
+        step_kernel<T, false, false, kAuxNt, true><<<grid, block_x, 0, stream>>>(p);
+      else
+        step_kernel<T, false, false, kAuxNt, false><<<grid, block_x, 0, stream>>>(p);
+    } else if (one) {
+      step_kernel<T, false, false, kAuxNone, true><<<grid, block_x, 0, stream>>>(p);
+    } else {
+      step_kernel<T, false, false, kAuxNone, false><<<grid, block_x, 0, stream>>>(p);
+    }
   }
   return hipGetLastError();
 }

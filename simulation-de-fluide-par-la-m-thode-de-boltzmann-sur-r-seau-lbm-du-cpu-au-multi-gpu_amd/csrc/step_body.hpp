@@ -23,8 +23,8 @@ struct DomainView {
 
 // Updates rows [y0, y1) of column x (one thread per column; every lane of a wave shares the
 // row, so row offsets stay in SGPRs). Lanes with x >= nx compute on column nx - 1 and store
-// nothing.
-template <class T, bool kOpen, bool kField>
+// nothing. kStoreAux: cache policy of the nine stores (kAuxNone or kAuxNt, streaming).
+template <class T, bool kOpen, bool kField, int kStoreAux = kAuxNone, class PS = Planes>
 __device__ __forceinline__ void update_column(const DomainView<T>& d, int x, int y0, int y1,
                                               const T* eq_in) {
   const int nx = d.nx;
@@ -37,7 +37,7 @@ __device__ __forceinline__ void update_column(const DomainView<T>& d, int x, int
   const Cols cols{(xc == 0 ? nx - 1 : xc - 1) * B, xc * B,
                   (xc + 1 == static_cast<unsigned>(nx) ? 0 : xc + 1) * B};
   const int seg = __builtin_amdgcn_readfirstlane(x) >> 6;  // wave-uniform 64-column segment
-  Planes g, gn;
+  PS g, gn;
   g.init(d.g, d.N);
   gn.init(d.gn, d.N);
   // Force planes (read only when kField; otherwise the resources are never used).
@@ -67,7 +67,7 @@ __device__ __forceinline__ void update_column(const DomainView<T>& d, int x, int
       }
       collide(v, fx, fy, d.omega, d.fp);
 #pragma unroll
-      for (int i = 0; i < kQ; ++i) Stored<T>::store(gn.r[i], cols.c, rows.c, v[i]);
+      for (int i = 0; i < kQ; ++i) gn.template store<T, kStoreAux>(i, cols.c, rows.c, v[i]);
     }
   }
 }

@@ -110,6 +110,7 @@ struct lbm_strip {
     p.tiles_x = (nx + block_x - 1) / block_x;
     p.nseg = nseg;
     p.wrap_y = 0;
+    p.one_rsrc = 1;  // used when the nine planes span < 4 GiB
     const T t = static_cast<T>(tau);
     p.omega = T(1) / t;
     p.fp = T(1) - T(0.5) * p.omega;

@@ -14,8 +14,15 @@ Cases (reference file:line they come from):
                 seed (numpy default_rng(seed): x0 in [0, nx-8], y0 in [1, ny-9]), rest init
   crop256_f64   256x256 walled channel, tau 0.6, gx 1e-6, 200 steps, rest init, double
                 (hash + sample only)
+  cylinder_probe_1024x256  the force-driven periodic channel with a staircase cylinder
+                (D 32 at (256, 129), Re 100 at U 0.05, Poiseuille init, 40,000 steps, float):
+                u_y at the wake probe (xc + 3D, yc) every 20 steps, as the reference's
+                cylinder driver samples it (src/common/driver.hpp:175-214), and the final
+                populations' SHA-256. Written by the oracle's OpenMP pull solver (the reference's
+                v0 takes no initial field; the oracle is pinned bitwise to v0 by the fixtures
+                above), since this case needs a non-uniform initial state.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [cylinder_probe]
 """
 from __future__ import annotations
 
@@ -70,7 +77,55 @@ def sample(a, k=257):
     return idx, a[idx]
 
 
+CYL = dict(nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0, Re=100.0, U=0.05, steps=40000,
+           every=20)
+
+
+def cylinder_probe_case():
+    """Inputs of the cylinder probe case: (tau, force, solid, initial AoS populations in float)."""
+    c = CYL
+    nx, ny, D = c["nx"], c["ny"], c["D"]
+    nu = c["U"] * D / c["Re"]
+    tau = 0.5 + 3.0 * nu
+    H = ny - 2
+    umax = 1.5 * c["U"]
+    force = 8.0 * nu * umax / (H * H)
+    solid = np.maximum(oracle.make_channel(nx, ny), oracle.make_cylinder(nx, ny, c["xc"], c["yc"], D))
+    yy = np.arange(ny, dtype=np.float64) - 0.5
+    prof = np.where((yy > 0) & (yy < H), 4 * umax * yy * (H - yy) / (H * H), 0.0)
+    u = np.repeat(prof[:, None], nx, axis=1).reshape(-1, 1)
+    cx = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], dtype=np.float64)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    cu = u * cx
+    S = (w * ((1.0 - 1.0) + 1.0 * (3 * cu + 4.5 * cu * cu - 1.5 * (u * u)))).astype(np.float32)
+    S[solid.astype(bool)] = 0
+    return tau, force, solid, S.reshape(-1)
+
+
+def cylinder_probe():
+    c = CYL
+    nx, ny = c["nx"], c["ny"]
+    tau, force, solid, S0 = cylinder_probe_case()
+    oracle.set_threads(8)
+    o = oracle.Oracle(nx, ny, tau, force, 0.0, solid, "float", "pull")
+    o.set_populations(S0)
+    px, py = int(c["xc"] + 3 * c["D"]), int(c["yc"])
+    cy = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], dtype=np.float64)
+    probe = []
+    for _ in range(c["steps"] // c["every"]):
+        o.run(c["every"])
+        S = o.populations()
+        f = S[(py * nx + px) * 9:(py * nx + px) * 9 + 9].astype(np.float64)
+        probe.append(float(cy @ f) / (1.0 + float(f.sum())))
+    np.savez_compressed(HERE / "cylinder_probe_1024x256_40000_float.npz", probe=np.array(probe),
+                        final_sha256=hashlib.sha256(S.tobytes()).hexdigest(),
+                        params=json.dumps(dict(c, tau=tau, force=force, probe=[px, py])))
+
+
 def main():
+    if sys.argv[1:] == ["cylinder_probe"]:
+        cylinder_probe()
+        return
     out = {}
     for prec in ("float", "double"):
         ref = oracle.Reference(prec)
@@ -109,6 +164,7 @@ def main():
                               total_mass=ref.L.ref_v0_total_mass(nx, ny, 0.6, 1e-6, 0.0, m, 1.0,
                                                                  0.0, 0.0, 200))
     (HERE / "hashes.json").write_text(json.dumps(out, indent=1))
+    cylinder_probe()
     print("golden fixtures written to", HERE)
 
 
