@@ -28,10 +28,18 @@ struct AdjointParams {
   double kd;           // the same in FP64, for the J partials
 };
 
-inline unsigned adjoint_blocks(size_t N) { return static_cast<unsigned>((N + 255) / 256); }
+// Block size of the reverse kernels, and tile width of the forward part of a fused launch.
+constexpr int kAdjointThreads = 256;
+
+__host__ __device__ inline unsigned adjoint_blocks(size_t N) {
+  return static_cast<unsigned>((N + kAdjointThreads - 1) / kAdjointThreads);
+}
 
 template <class T>
 hipError_t launch_adjoint_step(const AdjointParams<T>& p, hipStream_t stream);
+template <class T>
+hipError_t launch_adjoint_fused(const AdjointParams<T>& a, const StepParams<T>& f,
+                                hipStream_t stream);
 template <class T>
 hipError_t launch_probe(const GatherParams<T>& p, int px0, int py0, int pw, int ph, double k,
                         double* acc, hipStream_t stream);

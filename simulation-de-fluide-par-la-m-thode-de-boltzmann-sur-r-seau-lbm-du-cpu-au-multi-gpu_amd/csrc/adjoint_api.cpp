@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -85,7 +86,8 @@ struct lbm_adjoint {
   void* force = nullptr;
   void* dforce = nullptr;
   void* slots = nullptr;
-  void* work[2] = {nullptr, nullptr};
+  void* work[2] = {nullptr, nullptr};   // intermediate states of a recomputation
+  void* ready[2] = {nullptr, nullptr};  // recomputed R_t waiting for its reverse step
   void* mu[2] = {nullptr, nullptr};
   double* partials = nullptr;  // per reverse-kernel block: dJ/dtau, J
   double* jacc = nullptr;
@@ -95,6 +97,8 @@ struct lbm_adjoint {
   Revolve rv;
   long long forward_steps_scheduled = 0;
   int mu_cur = 0;  // index of the mu buffer holding mu_{t+1}
+  bool fuse = true;       // reverse steps share launches with forward steps (see forward_step)
+  bool use_graph = true;  // sweeps replayed as captured HIP graphs
   bool have_mu = false;
 
   size_t N() const { return static_cast<size_t>(nx) * ny; }
@@ -112,7 +116,7 @@ struct lbm_adjoint {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (void* p : {static_cast<void*>(links), static_cast<void*>(segflags), force, dforce, slots,
-                    work[0], work[1], mu[0], mu[1], static_cast<void*>(partials),
+                    work[0], work[1], ready[0], ready[1], mu[0], mu[1], static_cast<void*>(partials),
                     static_cast<void*>(jacc)})
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -121,7 +125,7 @@ struct lbm_adjoint {
   }
 
   template <class T>
-  int step(const void* src, void* dst) {
+  StepParams<T> step_params(const void* src, void* dst, int block_x, int rows) const {
     StepParams<T> p{};
     p.g = static_cast<const T*>(src);
     p.gn = static_cast<T*>(dst);
@@ -133,15 +137,20 @@ struct lbm_adjoint {
     p.ny = ny;
     p.row0 = 0;
     p.nrows = ny;
-    p.rows_per_tile = 1;
-    p.tiles_x = (nx + 63) / 64;
+    p.rows_per_tile = rows;
+    p.tiles_x = (nx + block_x - 1) / block_x;
     p.nseg = nseg;
     p.wrap_y = 1;
     p.one_rsrc = 1;
     const T t = static_cast<T>(tau);
     p.omega = T(1) / t;
     p.fp = T(1) - T(0.5) * p.omega;
-    const hipError_t e = launch_step(p, false, 64, stream);
+    return p;
+  }
+
+  template <class T>
+  int step(const void* src, void* dst) {
+    const hipError_t e = launch_step(step_params<T>(src, dst, 64, 1), false, 64, stream);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("adjoint forward: ") + hipGetErrorString(e));
     return LBM_OK;
   }
@@ -168,7 +177,7 @@ struct lbm_adjoint {
   int probe_any(const void* R) { return prec == LBM_DOUBLE ? probe<double>(R) : probe<float>(R); }
 
   template <class T>
-  int reverse(int t, const void* R) {
+  AdjointParams<T> reverse_params(int t, const void* R) const {
     AdjointParams<T> p{};
     p.R = static_cast<const T*>(R);
     p.mu_in = have_mu ? static_cast<const T*>(mu[mu_cur]) : nullptr;
@@ -190,25 +199,75 @@ struct lbm_adjoint {
     p.ph = ph;
     p.kd = 1.0 / (static_cast<double>(avg) * pw * ph);
     p.k = static_cast<T>(p.kd);
-    const hipError_t e = launch_adjoint_step(p, stream);
+    return p;
+  }
+
+  template <class T>
+  int reverse(int t, const void* R) {
+    const hipError_t e = launch_adjoint_step(reverse_params<T>(t, R), stream);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("adjoint step: ") + hipGetErrorString(e));
     mu_cur ^= 1;
     have_mu = true;
-
     return LBM_OK;
   }
   int reverse_any(int t, const void* R) {
     return prec == LBM_DOUBLE ? reverse<double>(t, R) : reverse<float>(t, R);
   }
 
-  // Advances from the state held in `src` by k >= 1 steps; the last step writes into `dst`.
+  // Reverse step t on the state held in R and, in the same launch, one forward step src -> dst.
+  template <class T>
+  int fused(int t, const void* R, const void* src, void* dst) {
+    const hipError_t e = launch_adjoint_fused(reverse_params<T>(t, R),
+                                              step_params<T>(src, dst, kAdjointThreads, 1), stream);
+    if (e != hipSuccess) return fail(LBM_EHIP, std::string("fused adjoint step: ") + hipGetErrorString(e));
+    mu_cur ^= 1;
+    have_mu = true;
+    return LBM_OK;
+  }
+  int fused_any(int t, const void* R, const void* src, void* dst) {
+    return prec == LBM_DOUBLE ? fused<double>(t, R, src, dst) : fused<float>(t, R, src, dst);
+  }
+
+  // A reverse step is issued lazily: it waits for the next operation and, when that is a
+  // forward step that does not overwrite the reverse step's input state, both share one launch
+  // (they touch disjoint buffers: the reverse step reads R_t and mu_{t+1} and writes mu_t and
+  // the gradient accumulators, which no forward step reads). Reverse steps keep their order.
+  struct Pending {
+    bool on = false;
+    int t = 0;
+    const void* R = nullptr;
+  } pending;
+  long long fused_launches = 0;
+
+  int flush() {
+    if (!pending.on) return LBM_OK;
+    pending.on = false;
+    return reverse_any(pending.t, pending.R);
+  }
+  int queue_reverse(int t, const void* R) {
+    LBM_TRY(flush());
+    pending = {true, t, R};
+    return LBM_OK;
+  }
+  int forward_step(const void* src, void* dst) {
+    forward_steps_scheduled++;
+    if (pending.on && fuse && dst != pending.R) {
+      pending.on = false;
+      fused_launches++;
+      return fused_any(pending.t, pending.R, src, dst);
+    }
+    LBM_TRY(flush());
+    return step_any(src, dst);
+  }
+
+  // Advances from the state held in `src` by k >= 1 steps; the last step writes into `dst`,
+  // the others into the work buffers (never the one `src` may be).
   int advance(const void* src, int k, void* dst) {
     const void* cur = src;
-    int w = 0;
+    int w = src == work[0] ? 1 : 0;
     for (int j = 0; j < k; ++j) {
       void* out = (j == k - 1) ? dst : work[w];
-      LBM_TRY(step_any(cur, out));
-      forward_steps_scheduled++;
+      LBM_TRY(forward_step(cur, out));
       cur = out;
       w ^= 1;
     }
@@ -219,16 +278,17 @@ struct lbm_adjoint {
   // s0 + nfree are free.
   int rev(int t0, int t1, int s0, int nfree) {
     const int n = t1 - t0;
-    if (n == 1) return reverse_any(t0, slot(s0));
+    if (n == 1) return queue_reverse(t0, slot(s0));
     if (nfree == 0) {
-      for (int t = t1 - 1; t > t0; --t) {
-        // recompute R_t from R_{t0} into a work buffer
-        const int k = t - t0;
-        void* dst = work[(k - 1) & 1];
-        LBM_TRY(advance(slot(s0), k, dst));
-        LBM_TRY(reverse_any(t, dst));
+      // No free slot: every R_t (t0 < t < t1) is recomputed from R_{t0}, alternately into the
+      // two ready buffers, so the recomputation of R_{t-1} never overwrites the input of the
+      // still-pending reverse step t.
+      int b = 0;
+      for (int t = t1 - 1; t > t0; --t, b ^= 1) {
+        LBM_TRY(advance(slot(s0), t - t0, ready[b]));
+        LBM_TRY(queue_reverse(t, ready[b]));
       }
-      return reverse_any(t0, slot(s0));
+      return queue_reverse(t0, slot(s0));
     }
     const int m = rv.first_split(n, nfree);
     LBM_TRY(advance(slot(s0), m, slot(s0 + 1)));
@@ -263,34 +323,41 @@ struct lbm_adjoint {
     return LBM_OK;
   }
 
-  int build_forward_graph() {
-    return capture(&fwd_exec, [&]() -> int {
-      LBM_TRY(prec == LBM_DOUBLE ? fills<double>() : fills<float>());
-      const void* cur = slot(0);
-      if (in_window(0)) LBM_TRY(probe_any(cur));
-      for (int t = 0; t < T; ++t) {
-        void* out = work[t & 1];
-        LBM_TRY(step_any(cur, out));
-        cur = out;
-        if (in_window(t + 1)) LBM_TRY(probe_any(cur));
-      }
-      return LBM_OK;
-    });
+  int forward_body() {
+    LBM_TRY(prec == LBM_DOUBLE ? fills<double>() : fills<float>());
+    const void* cur = slot(0);
+    if (in_window(0)) LBM_TRY(probe_any(cur));
+    for (int t = 0; t < T; ++t) {
+      void* out = work[t & 1];
+      LBM_TRY(step_any(cur, out));
+      cur = out;
+      if (in_window(t + 1)) LBM_TRY(probe_any(cur));
+    }
+    return LBM_OK;
   }
 
-  int build_gradient_graph() {
+  int gradient_body() {
     forward_steps_scheduled = 0;
-    return capture(&grad_exec, [&]() -> int {
-      LBM_TRY(prec == LBM_DOUBLE ? fills<double>() : fills<float>());
-      mu_cur = 0;
-      have_mu = false;
-      return rev(0, T + 1, 0, snapshots - 1);
-    });
+    LBM_TRY(prec == LBM_DOUBLE ? fills<double>() : fills<float>());
+    mu_cur = 0;
+    have_mu = false;
+    pending.on = false;
+    fused_launches = 0;
+    LBM_TRY(rev(0, T + 1, 0, snapshots - 1));
+    return flush();
   }
 
-  int run_graph(hipGraphExec_t exec, double* seconds) {
+  // Runs a sweep: replays its captured graph (built on first use), or, with
+  // LBM_ADJOINT_GRAPH=0, issues the launches directly (for kernel-level profiling).
+  int run(bool gradient, double* seconds) {
+    hipGraphExec_t* exec = gradient ? &grad_exec : &fwd_exec;
+    if (use_graph && !*exec)
+      LBM_TRY(capture(exec, [&]() -> int { return gradient ? gradient_body() : forward_body(); }));
     LBM_HIP_TRY(hipEventRecord(ev0, stream));
-    LBM_HIP_TRY(hipGraphLaunch(exec, stream));
+    if (use_graph)
+      LBM_HIP_TRY(hipGraphLaunch(*exec, stream));
+    else
+      LBM_TRY(gradient ? gradient_body() : forward_body());
     LBM_HIP_TRY(hipEventRecord(ev1, stream));
     LBM_HIP_TRY(hipEventSynchronize(ev1));
     float ms = 0.0f;
@@ -334,6 +401,13 @@ int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int ste
   a->ph = probe_h;
   a->avg = avg_steps;
   a->snapshots = std::min(snapshots, steps + 1);
+  // The fused launch addresses the nine planes through one buffer resource (< 4 GiB);
+  // LBM_ADJOINT_FUSE=0 turns it off (separate launches, same bits).
+  const char* fenv = std::getenv("LBM_ADJOINT_FUSE");
+  a->fuse = kQ * a->N() * real_size(precision) < (size_t(1) << 32) &&
+            !(fenv && std::string(fenv) == "0");
+  const char* genv = std::getenv("LBM_ADJOINT_GRAPH");
+  a->use_graph = !(genv && std::string(genv) == "0");
   a->solid.assign(solid, solid + a->N());
   a->nseg = (nx + 63) / 64;
   if (hipStreamCreate(&a->stream) != hipSuccess || hipEventCreate(&a->ev0) != hipSuccess ||
@@ -345,7 +419,8 @@ int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int ste
       hipMalloc(&a->force, 2 * a->N() * a->rs()) != hipSuccess ||
       hipMalloc(&a->dforce, 2 * a->N() * a->rs()) != hipSuccess ||
       hipMalloc(&a->slots, sb * a->snapshots) != hipSuccess || hipMalloc(&a->work[0], sb) != hipSuccess ||
-      hipMalloc(&a->work[1], sb) != hipSuccess || hipMalloc(&a->mu[0], sb) != hipSuccess ||
+      hipMalloc(&a->work[1], sb) != hipSuccess || hipMalloc(&a->ready[0], sb) != hipSuccess ||
+      hipMalloc(&a->ready[1], sb) != hipSuccess || hipMalloc(&a->mu[0], sb) != hipSuccess ||
       hipMalloc(&a->mu[1], sb) != hipSuccess ||
       hipMalloc(&a->partials, 2 * sizeof(double) * adjoint_blocks(a->N())) != hipSuccess ||
       hipMalloc(&a->jacc, sizeof(double)) != hipSuccess)
@@ -397,8 +472,7 @@ int lbm_adjoint_set_tau(lbm_adjoint* a, double tau) {
 
 int lbm_adjoint_forward(lbm_adjoint* a, double* J, double* seconds) {
   if (!a) return fail(LBM_EINVAL, "null adjoint handle");
-  if (!a->fwd_exec) LBM_TRY(a->build_forward_graph());
-  LBM_TRY(a->run_graph(a->fwd_exec, seconds));
+  LBM_TRY(a->run(false, seconds));
   if (J) LBM_HIP_TRY(hipMemcpy(J, a->jacc, sizeof(double), hipMemcpyDeviceToHost));
   return LBM_OK;
 }
@@ -406,8 +480,7 @@ int lbm_adjoint_forward(lbm_adjoint* a, double* J, double* seconds) {
 int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_dtau,
                          double* seconds, long long* forward_steps) {
   if (!a) return fail(LBM_EINVAL, "null adjoint handle");
-  if (!a->grad_exec) LBM_TRY(a->build_gradient_graph());
-  LBM_TRY(a->run_graph(a->grad_exec, seconds));
+  LBM_TRY(a->run(true, seconds));
   if (dJ_dforce)
     LBM_HIP_TRY(hipMemcpy(dJ_dforce, a->dforce, 2 * a->N() * a->rs(), hipMemcpyDeviceToHost));
   std::vector<double> part(2 * static_cast<size_t>(adjoint_blocks(a->N())));

@@ -5,13 +5,15 @@
 // Arithmetic (not bit-compatible with v0; checked against the FP64 step):
 //   * Moments and collision in scaled units (g' = S g): S drho = sum g', and the equilibrium
 //     w (S drho + rho S (3 cu + 4.5 cu^2 - 1.5 u^2)) needs no per-population rescaling.
-//   * Contracted to FMA, one reciprocal of rho; the Guo source with a uniform force reduces per
-//     direction to fma(B_i, c_i.u, A_i - C_i u.F) with per-direction constants.
-//   * Stores round stochastically to binary16: 13 random bits (a per-cell, per-step hash) are
-//     added to the fp32 pattern, which is then truncated to the 10-bit half mantissa and
-//     converted exactly. Round-to-nearest would be biased the same way in every cell of a
-//     uniform region and would stall the slow diffusive modes once their per-step change drops
-//     below half an ulp; the unbiased rounding keeps the expected update exact.
+//   * Contracted to FMA, one reciprocal of rho; omega w and omega w S rho folded per weight class,
+//     so each direction costs the polynomial, one FMA for omega geq, and the Guo source
+//     fma(B_i, c_i.u, A_i - C_k u.F); the constants come precomputed with the launch.
+//   * Stores round stochastically to binary16 with the hardware conversion v_cvt_sr_f16_f32
+//     (its rounding threshold comes from the high bits of a 32-bit random operand): a per-cell,
+//     per-step hash, advanced by the golden-ratio increment for each of the nine values.
+//     Round-to-nearest would be biased the same way in every cell of a uniform region and would
+//     stall the slow diffusive modes once their per-step change drops below half an ulp; the
+//     unbiased rounding keeps the expected update exact.
 //   * The nine roundings of a cell are chained so that its mass and momentum carry only the
 //     noise of the finest values: the axis directions are rounded first and their momentum
 //     error is taken off the diagonals (a mass-neutral shift), then the mass error of the eight
@@ -35,17 +37,12 @@ __device__ __forceinline__ uint32_t hash_cell(uint32_t c, uint32_t seed) {
   return h;
 }
 
-__device__ __forceinline__ uint32_t xorshift(uint32_t x) {
-  x ^= x << 13;
-  x ^= x >> 17;
-  x ^= x << 5;
-  return x;
-}
+using half2_t = _Float16 __attribute__((ext_vector_type(2)));
 
-// fp32 (scaled units) -> binary16 bits, stochastic rounding with 13 random bits.
-__device__ __forceinline__ unsigned short to_half_sr(float v, uint32_t r13) {
-  const uint32_t b = (__builtin_bit_cast(uint32_t, v) + r13) & ~0x1FFFu;
-  return __builtin_bit_cast(unsigned short, static_cast<_Float16>(__builtin_bit_cast(float, b)));
+// fp32 (scaled units) -> binary16 bits, stochastic rounding driven by the high bits of r.
+__device__ __forceinline__ unsigned short to_half_sr(float v, uint32_t r) {
+  const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{}, v, r, false);
+  return __builtin_bit_cast(unsigned short, h[0]);
 }
 
 __device__ __forceinline__ float half_value(unsigned short h) {
@@ -56,16 +53,13 @@ __device__ __forceinline__ float half_value(unsigned short h) {
 // Rounding errors e = half(v) - v are exact in fp32 (v and its rounding are within a factor 2).
 __device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
                                            unsigned short (&h)[kQ]) {
-  uint32_t r13[kQ];
+  uint32_t rnd[kQ];
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    if (i == 2 || i == 4 || i == 6 || i == 8) r = xorshift(r);
-    r13[i] = (i & 1) ? (r >> 13) & 0x1FFFu : r & 0x1FFFu;
-  }
+  for (int i = 0; i < kQ; ++i) rnd[i] = r + static_cast<uint32_t>(i) * 0x9E3779B9u;
   float e[kQ];
 #pragma unroll
   for (int i = 1; i <= 4; ++i) {
-    h[i] = to_half_sr(v[i], r13[i]);
+    h[i] = to_half_sr(v[i], rnd[i]);
     e[i] = half_value(h[i]) - v[i];
   }
   const float ex = e[1] - e[3], ey = e[2] - e[4];
@@ -73,57 +67,45 @@ __device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
 #pragma unroll
   for (int i = 5; i < kQ; ++i) {  // sum_d c_d (-(ex c_dx + ey c_dy) / 4) = -(ex, ey), sum_d = 0
     const float t = v[i] - 0.25f * (kCx[i] * ex + kCy[i] * ey);
-    h[i] = to_half_sr(t, r13[i]);
+    h[i] = to_half_sr(t, rnd[i]);
     em += half_value(h[i]) - v[i];
   }
-  h[0] = to_half_sr(v[0] - em, r13[0]);
+  h[0] = to_half_sr(v[0] - em, rnd[0]);
 }
 
-struct FastCollide {
-  float omega, S, invS, hfx, hfy, fx, fy;
-  float A[kQ], B[kQ], C[kQ];  // Guo: src'_i = B_i cu_i + A_i - C_i u.F (scaled units)
+// Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
+__device__ __forceinline__ constexpr int wclass(int i) { return i == 0 ? 0 : (i < 5 ? 1 : 2); }
 
-  __device__ __forceinline__ void setup(float omega_, float fp, float S_, float invS_, float fx_,
-                                        float fy_) {
-    omega = omega_;
-    S = S_;
-    invS = invS_;
-    fx = fx_;
-    fy = fy_;
-    hfx = 0.5f * fx_;
-    hfy = 0.5f * fy_;
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const float w = weight<float>(i);
-      const float cf = kCx[i] * fx_ + kCy[i] * fy_;
-      A[i] = 3.0f * S * fp * w * cf;
-      B[i] = 9.0f * S * fp * w * cf;
-      C[i] = 3.0f * S * fp * w;
-    }
-  }
-
-  __device__ __forceinline__ void operator()(float (&v)[kQ]) const {
+// BGK + Guo in scaled units with the constants of HalfCollide.
+__device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ]) {
 #pragma clang fp contract(fast)
-    const float m0 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])) + v[8];
-    const float mx = (v[1] + v[5] + v[8]) - (v[3] + v[6] + v[7]);
-    const float my = (v[2] + v[5] + v[6]) - (v[4] + v[7] + v[8]);
-    const float rho = 1.0f + m0 * invS;
-    const float ir = __builtin_amdgcn_rcpf(rho);
-    const float ux = (mx * invS + hfx) * ir;
-    const float uy = (my * invS + hfy) * ir;
-    const float usq15 = 1.5f * (ux * ux + uy * uy);
-    const float rhoS = rho * S;
-    const float uF = ux * fx + uy * fy;
+  const float m0 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])) + v[8];
+  const float d57 = v[5] - v[7], d86 = v[8] - v[6];
+  const float mx = ((v[1] - v[3]) + d57) + d86;
+  const float my = ((v[2] - v[4]) + d57) - d86;
+  const float rho = 1.0f + m0 * c.invS;
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  const float ux = (mx * c.invS + c.hfx) * ir;
+  const float uy = (my * c.invS + c.hfy) * ir;
+  const float usq15 = 1.5f * (ux * ux + uy * uy);
+  const float uF = ux * c.fx + uy * c.fy;
+  float a[3], b[3], cuf[3];  // per class: omega w m0, omega w S rho, C_k u.F
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const float cu = kCx[i] * ux + kCy[i] * uy;
-      const float poly = (4.5f * cu + 3.0f) * cu - usq15;
-      const float geq = weight<float>(i) * (m0 + rhoS * poly);
-      const float src = B[i] * cu + (A[i] - C[i] * uF);
-      v[i] = v[i] + omega * (geq - v[i]) + src;
-    }
+  for (int k = 0; k < 3; ++k) {
+    a[k] = c.ow[k] * m0;
+    b[k] = c.owS[k] * rho;
+    cuf[k] = c.C[k] * uF;
   }
-};
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const int k = wclass(i);
+    const float cu = kCx[i] * ux + kCy[i] * uy;
+    const float poly = (4.5f * cu + 3.0f) * cu - usq15;
+    const float ogeq = b[k] * poly + a[k];
+    const float src = c.B[i] * cu + (c.A[i] - cuf[k]);
+    v[i] = (c.om1 * v[i] + ogeq) + src;
+  }
+}
 
 template <bool kOpen, bool kField, bool kOne>
 __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
@@ -145,8 +127,6 @@ __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
   PS g, gn;
   g.init(p.g, p.N);
   gn.init(p.gn, p.N);
-  FastCollide col;
-  col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
 
   for (int y = y0; y < y1; ++y) {
     int ym = y - 1, yp = y + 1;
@@ -167,11 +147,11 @@ __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
     if (active) {
       const uint32_t cell = static_cast<uint32_t>(y) * static_cast<uint32_t>(nx) + xc;
       if constexpr (kField) {
-        FastCollide c2 = col;
+        HalfCollide c2;
         c2.setup(p.omega, p.fp, p.scale, p.inv_scale, p.force[cell], p.force[p.N + cell]);
-        c2(v);
+        fast_collide(c2, v);
       } else {
-        col(v);
+        fast_collide(p.col, v);
       }
       unsigned short h[kQ];
       round_cell(v, hash_cell(cell, p.seed), h);

@@ -18,6 +18,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "d2q9.hpp"
+
 namespace lbmhip {
 
 template <class T>
@@ -40,6 +42,37 @@ struct StepParams {
   int one_rsrc;        // 1: one buffer resource for the nine planes when they fit in 4 GiB
 };
 
+// Collision constants of the FP16-storage step (scaled units g' = S g), computed once per launch
+// on the host (and per cell on the device for a per-cell force field):
+//   omega geq_i = b_k poly_i + a_k with a_k = omega w_k m0', b_k = omega w_k S rho (class k),
+//   Guo source src'_i = B_i c_i.u + A_i - C_k u.F.
+struct HalfCollide {
+  float om1, invS, hfx, hfy, fx, fy;
+  float ow[3], owS[3];  // omega w_k, omega w_k S for the rest, axis and diagonal classes
+  float A[9], B[9], C[3];
+
+  __host__ __device__ void setup(float omega, float fp, float S, float inv_s, float fx_, float fy_) {
+    om1 = 1.0f - omega;
+    invS = inv_s;
+    fx = fx_;
+    fy = fy_;
+    hfx = 0.5f * fx_;
+    hfy = 0.5f * fy_;
+    for (int i = 0; i < kQ; ++i) {
+      const float w = weight<float>(i);
+      const float cf = kCx[i] * fx_ + kCy[i] * fy_;
+      A[i] = 3.0f * S * fp * w * cf;
+      B[i] = 9.0f * S * fp * w * cf;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const float w = weight<float>(k == 0 ? 0 : (k == 1 ? 1 : 5));
+      ow[k] = omega * w;
+      owS[k] = omega * w * S;
+      C[k] = 3.0f * S * fp * w;
+    }
+  }
+};
+
 // FP16-storage step: halves hold g * scale; arithmetic in fp32 (fp16.hip).
 struct StepParamsH {
   const uint16_t* g;
@@ -58,6 +91,7 @@ struct StepParamsH {
   float scale, inv_scale;
   float eq_in[9];  // inlet equilibrium, already scaled
   uint32_t seed;   // stochastic-rounding stream of this step
+  HalfCollide col; // uniform-force collision constants
 };
 
 // Read-back parameters (gather of the post-streaming state), float / double storage.

@@ -108,6 +108,7 @@ StepParamsH make_step_params_h(const SolverState& s) {
   if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i] * p.scale;
   p.seed = s.step_count * 0x9E3779B9u + 0x7F4A7C15u;
+  p.col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
   return p;
 }
 
@@ -134,6 +135,18 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
   if (storage_size(prec) * cells() >= (size_t(1) << 32))
     return fail(LBM_EINVAL, "one population plane must stay below 4 GiB");
   solid.assign(solid_, solid_ + cells());
+  // Launch defaults from a shape scan of the 8192^2 channel on MI355X: one row per thread and
+  // non-temporal stores for fp32/fp64 (512 x 1: more waves in flight, no per-wave row loop);
+  // FP16 prefers 1024 x 2 with ordinary stores. LBM_BLOCK overrides the shape.
+  if (prec == LBM_FP16) {
+    block_x = 1024;
+    rows = 2;
+    store_nt = 0;
+  } else {
+    block_x = 512;
+    rows = 1;
+    store_nt = 1;
+  }
   LBM_TRY(parse_block_env(&block_x, &rows));
   LBM_HIP_TRY(hipStreamCreate(&stream));
   LBM_HIP_TRY(hipEventCreate(&ev0));
@@ -407,12 +420,13 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
   LBM_HIP_TRY(hipStreamSynchronize(stream));
   LBM_HIP_TRY(hipMemcpy(backup, g, bytes, hipMemcpyDeviceToDevice));
   const int keep_bx = block_x, keep_rows = rows;
+  const uint32_t keep_count = step_count;  // FP16 rounding seeds stay as if no trial ran
   double best = -1.0;
   int bbx = keep_bx, brows = keep_rows, bnt = store_nt, bone = one_rsrc;
   for (int one : {1, 0})
   for (int nt : {0, 1})
-  for (int bx : {64, 128, 256, 512, 1024})
-    for (int r : {1, 2, 4, 8, 16, 32}) {
+  for (int bx : {128, 256, 512, 1024})
+    for (int r : {1, 2, 4, 8, 16}) {
       block_x = bx;
       rows = r;
       store_nt = nt;
@@ -438,6 +452,7 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
   rows = brows;
   store_nt = bnt;
   one_rsrc = bone;
+  step_count = keep_count;
   LBM_HIP_TRY(hipMemcpy(g, backup, bytes, hipMemcpyDeviceToDevice));
   LBM_HIP_TRY(hipMemcpy(gn, backup, bytes, hipMemcpyDeviceToDevice));
   LBM_HIP_TRY(hipFree(backup));
@@ -481,11 +496,12 @@ int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
     }
     return best;
   };
-  // The reference's scalar kernel (grid = CUs x 32, block 256) and 16-byte variants over a few
-  // grid sizes; the ceiling is the best of all (a higher ceiling is the conservative choice).
+  // The reference's scalar kernel (grid = CUs x 32, block 256) and 16-byte variants over grid
+  // sizes up to about one 16-byte access per lane (the largest grids keep the most bytes in
+  // flight); the ceiling is the best of all (a higher ceiling is the conservative choice).
   double kms = 1e30;
   for (int variant : {0, 1, 2})
-    for (int per_cu : {8, 32, 128}) {
+    for (int per_cu : {8, 32, 128, 512, 2048}) {
       kms = std::min(kms, best_ms([&] { (void)launch_copy(a, b, n, cus * per_cu, 256, variant, st); }));
       LBM_HIP_TRY(hipGetLastError());
     }

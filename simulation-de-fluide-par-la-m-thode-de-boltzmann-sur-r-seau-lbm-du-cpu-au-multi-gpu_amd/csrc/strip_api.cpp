@@ -31,7 +31,7 @@ struct lbm_strip {
   int nx = 0, nyl = 0, prec = 0;
   double tau = 0.8, fx = 0.0, fy = 0.0, u_in = 0.0;
   bool open_x = false;
-  int block_x = 256, rows = 8;
+  int block_x = 512, rows = 1;  // the plain solver's fp32/fp64 defaults (LBM_BLOCK overrides)
   std::vector<uint8_t> solid;  // (nyl + 2) * nx, ghost rows included
   void* g = nullptr;
   void* gn = nullptr;
@@ -111,6 +111,7 @@ struct lbm_strip {
     p.nseg = nseg;
     p.wrap_y = 0;
     p.one_rsrc = 1;  // used when the nine planes span < 4 GiB
+    p.store_nt = 1;  // non-temporal stores (periodic-x, uniform-force kernel)
     const T t = static_cast<T>(tau);
     p.omega = T(1) / t;
     p.fp = T(1) - T(0.5) * p.omega;
