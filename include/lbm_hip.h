@@ -78,12 +78,15 @@ int lbm_set_tau(lbm_solver* s, double tau);
  * 145-153, is read from the environment at creation with the same syntax). */
 int lbm_set_block(lbm_solver* s, int block_x, int rows_per_thread);
 int lbm_get_block(const lbm_solver* s, int* block_x, int* rows_per_thread);
-/* No reference counterpart: code-generation options of the periodic, uniform-force step (same
- * bits either way; lbm_autotune tries them): streaming_stores = non-temporal stores;
- * single_resource = one buffer resource for the nine planes when they span < 4 GiB (fewer
- * scalar registers, higher occupancy). Defaults: 0, 1. */
-int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource);
-int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource);
+/* No reference counterpart: code-generation options of the step (same bits either way;
+ * lbm_autotune tries them): streaming_stores = non-temporal stores (periodic, uniform-force
+ * kernel); single_resource = one buffer resource for the nine planes when they span < 4 GiB
+ * (fewer scalar registers); xcd_bands = workgroups remapped so that each of the 8 XCDs walks
+ * a contiguous band of tiles (0: the hardware's round-robin order). Defaults: fp32 1, 1, 0;
+ * fp64 1, 1, 1; FP16 0, 1, 0. */
+int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int xcd_bands);
+int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
+                           int* xcd_bands);
 
 /* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */
 int lbm_step(lbm_solver* s);

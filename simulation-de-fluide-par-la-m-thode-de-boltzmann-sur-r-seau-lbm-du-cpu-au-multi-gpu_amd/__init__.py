@@ -72,8 +72,8 @@ def _declare(L):
         "lbm_set_tau": ([vp, d], i),
         "lbm_set_block": ([vp, i, i], i),
         "lbm_get_block": ([vp, ip, ip], i),
-        "lbm_set_kernel_options": ([vp, i, i], i),
-        "lbm_get_kernel_options": ([vp, ip, ip], i),
+        "lbm_set_kernel_options": ([vp, i, i, i], i),
+        "lbm_get_kernel_options": ([vp, ip, ip, ip], i),
         "lbm_step": ([vp], i),
         "lbm_run": ([vp, i], i),
         "lbm_timed_run": ([vp, i, dp], i),
@@ -357,14 +357,15 @@ class Solver:
     def set_block(self, block_x, rows_per_thread):
         check(lib().lbm_set_block(self._h, int(block_x), int(rows_per_thread)))
 
-    def set_kernel_options(self, streaming_stores: bool, single_resource: bool):
+    def set_kernel_options(self, streaming_stores: bool, single_resource: bool,
+                           xcd_bands: bool = False):
         check(lib().lbm_set_kernel_options(self._h, int(bool(streaming_stores)),
-                                           int(bool(single_resource))))
+                                           int(bool(single_resource)), int(bool(xcd_bands))))
 
-    def kernel_options(self) -> tuple[bool, bool]:
-        a, b = C.c_int(), C.c_int()
-        check(lib().lbm_get_kernel_options(self._h, C.byref(a), C.byref(b)))
-        return bool(a.value), bool(b.value)
+    def kernel_options(self) -> tuple[bool, bool, bool]:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().lbm_get_kernel_options(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return bool(a.value), bool(b.value), bool(c.value)
 
     def block(self) -> tuple[int, int]:
         bx, r = C.c_int(), C.c_int()

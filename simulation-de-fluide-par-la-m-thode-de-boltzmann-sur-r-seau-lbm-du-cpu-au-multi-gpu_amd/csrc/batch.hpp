@@ -32,7 +32,11 @@ struct BatchParams {
   const BatchTile* tiles;
   int ntiles;
   int tile_rows;
-  int chunk;  // tiles per workgroup
+  int chunk;     // order 1: tiles per workgroup
+  int order;     // 0: workgroup w takes tiles w, w + G, w + 2G, ... (G = grid size);
+                 // 1: a contiguous chunk of tiles, workgroups in per-XCD bands
+  int store_nt;  // 1: non-temporal stores
+  int one_rsrc;  // 1: every domain's nine planes span < 4 GiB (one buffer resource each)
 };
 
 template <class T>

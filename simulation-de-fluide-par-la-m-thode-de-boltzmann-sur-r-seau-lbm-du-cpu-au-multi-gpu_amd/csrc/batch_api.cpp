@@ -44,6 +44,8 @@ struct lbm_batch {
   size_t tiles_cap = 0;
   int ntiles = 0;
   int tile_x = 256, tile_rows = 4, workgroups = 0;
+  int order = 0, store_nt = 1;  // tile order (see batch.hip) and non-temporal stores
+  bool one_rsrc = true;         // every domain's nine planes span < 4 GiB
   int graph_steps = 100;
   int parity = 0;  // 0: current state in g
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -130,6 +132,9 @@ struct lbm_batch {
     p.tile_rows = tile_rows;
     const int wg = std::max(1, std::min(workgroups, ntiles));
     p.chunk = (ntiles + wg - 1) / wg;
+    p.order = order;
+    p.store_nt = store_nt;
+    p.one_rsrc = one_rsrc ? 1 : 0;
     return p;
   }
 
@@ -222,6 +227,7 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
     b->total_elems += (kQ * h.N() + kAlignElems - 1) / kAlignElems * kAlignElems;
     b->total_cells += h.N();
     b->total_segs += static_cast<size_t>(h.ny) * h.nseg;
+    if (kQ * h.N() * b->rs() >= (size_t(1) << 32)) b->one_rsrc = false;
   }
   int rc = parse_block_env(&b->tile_x, &b->tile_rows);
   if (rc != LBM_OK) return bail(rc);
@@ -383,13 +389,18 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   cus = std::max(cus, 1);
   double best = -1.0;
-  int btx = b->tile_x, brows = b->tile_rows, bwg = b->workgroups;
-  for (int tx : {64, 128, 256, 512})
-    for (int r : {1, 2, 4, 8, 16})
-      for (int wgm : {2, 4, 8, 16}) {
+  int btx = b->tile_x, brows = b->tile_rows, bwg = b->workgroups, border = b->order,
+      bnt = b->store_nt;
+  for (int ord : {0, 1})
+  for (int nt : {1, 0})
+  for (int tx : {128, 256, 512})
+    for (int r : {1, 2, 4, 8})
+      for (int wgm : {4, 8, 16, 32, 64}) {
         b->tile_x = tx;
         b->tile_rows = r;
         b->workgroups = cus * wgm;
+        b->order = ord;
+        b->store_nt = nt;
         LBM_TRY(b->build_tiles());
         LBM_TRY(b->step_direct());  // warm-up
         LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
@@ -404,11 +415,15 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
           btx = tx;
           brows = r;
           bwg = cus * wgm;
+          border = ord;
+          bnt = nt;
         }
       }
   b->tile_x = btx;
   b->tile_rows = brows;
   b->workgroups = bwg;
+  b->order = border;
+  b->store_nt = bnt;
   LBM_TRY(b->build_tiles());
   b->parity = keep_parity;
   LBM_HIP_TRY(hipMemcpy(b->current(), backup, bytes, hipMemcpyDeviceToDevice));

@@ -40,6 +40,7 @@ struct StepParams {
   T eq_in[9];          // inlet equilibrium (open x)
   int store_nt;        // 1: non-temporal stores (periodic, uniform-force kernel only)
   int one_rsrc;        // 1: one buffer resource for the nine planes when they fit in 4 GiB
+  int xcd_bands;       // 1: workgroup ids remapped to contiguous tile bands per XCD
 };
 
 // Collision constants of the FP16-storage step (scaled units g' = S g), computed once per launch
@@ -91,6 +92,7 @@ struct StepParamsH {
   float scale, inv_scale;
   float eq_in[9];  // inlet equilibrium, already scaled
   uint32_t seed;   // stochastic-rounding stream of this step
+  int xcd_bands;   // 1: workgroup ids remapped to contiguous tile bands per XCD
   HalfCollide col; // uniform-force collision constants
 };
 

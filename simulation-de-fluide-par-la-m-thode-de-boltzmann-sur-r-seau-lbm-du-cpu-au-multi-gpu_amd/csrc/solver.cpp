@@ -117,17 +117,20 @@ int lbm_set_block(lbm_solver* s, int block_x, int rows_per_thread) {
   return LBM_OK;
 }
 
-int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource) {
+int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int xcd_bands) {
   LBM_TRY(check_handle(s));
   s->store_nt = streaming_stores ? 1 : 0;
   s->one_rsrc = single_resource ? 1 : 0;
+  s->xcd_bands = xcd_bands ? 1 : 0;
   return LBM_OK;
 }
 
-int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource) {
+int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
+                           int* xcd_bands) {
   LBM_TRY(check_handle(s));
   if (streaming_stores) *streaming_stores = s->store_nt;
   if (single_resource) *single_resource = s->one_rsrc;
+  if (xcd_bands) *xcd_bands = s->xcd_bands;
   return LBM_OK;
 }
 
@@ -240,9 +243,11 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
     std::snprintf(buf, len,
                   "\"gpu\": \"%s\", \"arch\": \"%s\", \"hip_runtime\": %d, \"hip_driver\": %d, "
                   "\"block\": \"%dx%d\", \"variant\": \"soa_pull_buffer\", \"storage\": \"%s\", "
-                  "\"stores\": \"%s\"",
+                  "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\"",
                   prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
-                  s->store_nt ? "nontemporal" : "default");
+                  s->store_nt ? "nontemporal" : "default",
+                  s->one_rsrc ? "one_resource" : "resource_per_plane",
+                  s->xcd_bands ? "xcd_bands" : "round_robin");
   return LBM_OK;
 }
 

@@ -17,6 +17,7 @@ struct SolverState {
   int block_x = 256, rows = 8;  // launch shape (tile = block_x columns x rows)
   int store_nt = 0;             // non-temporal stores in the periodic uniform-force step
   int one_rsrc = 1;             // one buffer resource for all planes (when < 4 GiB)
+  int xcd_bands = 0;            // 1: each XCD walks a contiguous band of tiles
   double fp16_scale = 4096.0;   // stored half = g * scale (LBM_FP16)
   uint32_t step_count = 0;      // steps taken (seeds the FP16 stochastic rounding)
 

@@ -10,7 +10,8 @@
 //     rows. Row indices are wave-uniform: the row offsets live in SGPRs (buffer soffset) and each
 //     lane only carries three column byte offsets (x - 1, x, x + 1, wrapped once per thread with
 //     a compare, never a modulo).
-//   * Workgroup ids are remapped so that each of the 8 XCDs walks a contiguous band of tiles.
+//   * Workgroups run in the hardware's round-robin XCD order by default; xcd_bands remaps them
+//     so that each of the 8 XCDs walks a contiguous band of tiles (measured no faster here).
 //   * Solid cells, bounce-back links and open-x edges are handled only in "dirty" 64-column
 //     segments (one scalar flag per wave and row); clean segments, the bulk of a channel, read
 //     nothing but the nine populations: 72 B per cell update in fp32, the algorithmic minimum.
@@ -26,7 +27,7 @@ namespace {
 
 template <class T, bool kOpen, bool kField, int kStoreAux, bool kOne = false>
 __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned tile = p.xcd_bands ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);

@@ -112,6 +112,7 @@ struct lbm_strip {
     p.wrap_y = 0;
     p.one_rsrc = 1;  // used when the nine planes span < 4 GiB
     p.store_nt = 1;  // non-temporal stores (periodic-x, uniform-force kernel)
+    p.xcd_bands = 0; // hardware round-robin workgroup order
     const T t = static_cast<T>(tau);
     p.omega = T(1) / t;
     p.fp = T(1) - T(0.5) * p.omega;
