@@ -85,12 +85,17 @@ def cpu_baseline(nx, ny, steps=10, warmup=2):
 
 def form_copy_ceiling(lbm, args):
     nbytes = 2_415_919_104  # one fp32 population buffer of the 8192^2 grid (>> 256 MB MALL)
-    copy_gbs, memcpy_gbs = lbm.copy_ceiling(nbytes, 20)
-    ceiling = max(copy_gbs, memcpy_gbs)
-    return {"copy_kernel_gbs": round(copy_gbs, 1), "memcpy_dtod_gbs": round(memcpy_gbs, 1),
+    c = lbm.copy_ceiling(nbytes, 20)
+    ceiling = max(c["copy_kernel"], c["memcpy"])
+    return {"copy_kernel_gbs": round(c["copy_kernel"], 1), "memcpy_dtod_gbs": round(c["memcpy"], 1),
+            "reference_method_copy_gbs": round(c["reference_kernel"], 1),
+            "nine_array_copy_gbs": round(c["nine_array_copy"], 1),
             "bytes_per_array": nbytes, "reps": 20,
             "mlups_ceiling_fp32": round(ceiling * 1e3 / 72, 1),
-            "mlups_ceiling_fp16": round(ceiling * 1e3 / 36, 1)}
+            "mlups_ceiling_fp16": round(ceiling * 1e3 / 36, 1),
+            "note": "ceiling = max(best copy kernel, memcpy); the reference's own copy "
+                    "configuration and a nine-array copy (the planes' stream structure) are "
+                    "reported beside it"}
 
 
 def form_precisions(lbm, args):
@@ -271,9 +276,13 @@ def main():
     }
     if not args.no_forms:
         line["forms"] = forms(lbm, args)
-        ceil = line["forms"].get("copy_ceiling", {}).get("mlups_ceiling_fp32")
-        if ceil:
-            line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / ceil, 4)
+        cc = line["forms"].get("copy_ceiling", {})
+        if cc.get("mlups_ceiling_fp32"):
+            line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / cc["mlups_ceiling_fp32"], 4)
+            line["forms"]["plain_float_frac_of_reference_method_copy"] = round(
+                achieved / cc["reference_method_copy_gbs"], 4)
+            line["forms"]["plain_float_frac_of_nine_array_copy"] = round(
+                achieved / cc["nine_array_copy_gbs"], 4)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(args.nx, args.ny, steps=args.cpu_steps)

@@ -212,9 +212,17 @@ int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_
 int lbm_checkpoint_forward_steps(int states, int slots, long long* steps);
 
 /* ---- HBM copy ceiling (the counterpart of bench/gpu_bandwidth.cu) ---------------------------- */
-/* Best-of-reps copy bandwidth (GB/s, 2 * bytes / time) of a grid-stride copy kernel and of
- * hipMemcpyDtoD on two arrays of `bytes` each. */
-int lbm_copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs);
+/* Best-of-reps copy bandwidth (GB/s, 2 * bytes / time) on two arrays of `bytes` each:
+ *   copy_gbs      best grid-stride copy kernel (scalar, 16-byte and 16-byte non-temporal
+ *                 accesses, grids of 8..2048 workgroups per CU);
+ *   memcpy_gbs    hipMemcpyDtoD;
+ *   reference_gbs the reference's own configuration (scalar, block 256, grid = CUs x 32,
+ *                 bench/gpu_bandwidth.cu:17-22,66-67);
+ *   nine_gbs      nine arrays in, nine out, one element of each per thread (the stream
+ *                 structure of the step's nine population planes).
+ * Any output pointer may be null. */
+int lbm_copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
+                     double* reference_gbs, double* nine_gbs);
 
 #ifdef __cplusplus
 }

@@ -92,7 +92,7 @@ def _declare(L):
         "lbm_autotune": ([vp, i, ip, ip, dp], i),
         "lbm_poiseuille_force": ([i, i, d, d], d),
         "lbm_momentum_exchange": ([vp, i, i, i, i, dp, dp], i),
-        "lbm_copy_ceiling": ([sz, i, dp, dp], i),
+        "lbm_copy_ceiling": ([sz, i, dp, dp, dp, dp], i),
         "lbm_strip_create": ([i, i, i, C.POINTER(vp)], i),
         "lbm_strip_destroy": ([vp], i),
         "lbm_strip_set_params": ([vp, d, d, d, i, d], i),
@@ -173,11 +173,15 @@ def poiseuille_force(H: int, tau: float, u_max: float, precision="float") -> flo
     return lib().lbm_poiseuille_force(precision_code(precision), int(H), float(tau), float(u_max))
 
 
-def copy_ceiling(nbytes: int = 1 << 32, reps: int = 20) -> tuple[float, float]:
-    """HBM copy ceiling (GB/s): grid-stride copy kernel and hipMemcpyDtoD, best of reps."""
-    a, b = C.c_double(), C.c_double()
-    check(lib().lbm_copy_ceiling(int(nbytes), int(reps), C.byref(a), C.byref(b)))
-    return a.value, b.value
+def copy_ceiling(nbytes: int = 1 << 32, reps: int = 20) -> dict:
+    """HBM copy ceilings (GB/s, best of reps, two arrays of nbytes): the best grid-stride copy
+    kernel, hipMemcpyDtoD, the reference's own copy configuration (bench/gpu_bandwidth.cu:
+    17-22,66-67), and a nine-array copy (the stream structure of nine population planes)."""
+    a, b, r, n = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    check(lib().lbm_copy_ceiling(int(nbytes), int(reps), C.byref(a), C.byref(b), C.byref(r),
+                                 C.byref(n)))
+    return {"copy_kernel": a.value, "memcpy": b.value, "reference_kernel": r.value,
+            "nine_array_copy": n.value}
 
 
 # ---- geometry (src/common/geometry.cpp) ------------------------------------------------------

@@ -156,5 +156,8 @@ hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, 
 // variant 0: scalar (the reference's kernel), 1: 16-byte accesses, 2: 16-byte non-temporal.
 hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block, int variant,
                        hipStream_t stream);
+// Nine-array copy: b[k * n9 + i] = a[k * n9 + i] for k < 9, one i per thread (non-temporal
+// when nt), over 9 * n9 floats.
+hipError_t launch_copy9(const float* a, float* b, size_t n9, int nt, hipStream_t stream);
 
 }  // namespace lbmhip

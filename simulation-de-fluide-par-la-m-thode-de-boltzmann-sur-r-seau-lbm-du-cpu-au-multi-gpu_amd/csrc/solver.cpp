@@ -282,8 +282,9 @@ double lbm_poiseuille_force(int precision, int H, double tau, double u_max) {
   return static_cast<double>(8.0f * nu * u / (float(H) * float(H)));
 }
 
-int lbm_copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
-  return lbmhip::copy_ceiling(bytes, reps, copy_gbs, memcpy_gbs);
+int lbm_copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
+                     double* reference_gbs, double* nine_gbs) {
+  return lbmhip::copy_ceiling(bytes, reps, copy_gbs, memcpy_gbs, reference_gbs, nine_gbs);
 }
 
 }  // extern "C"

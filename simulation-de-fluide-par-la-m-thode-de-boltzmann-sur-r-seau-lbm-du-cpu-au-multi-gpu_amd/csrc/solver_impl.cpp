@@ -472,7 +472,8 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
   return LBM_OK;
 }
 
-int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
+int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
+                 double* reference_gbs, double* nine_gbs) {
   if (bytes < 4 || reps < 1) return fail(LBM_EINVAL, "need bytes >= 4 and reps >= 1");
   const size_t n = bytes / sizeof(float);
   float *a = nullptr, *b = nullptr;
@@ -515,6 +516,11 @@ int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
       kms = std::min(kms, best_ms([&] { (void)launch_copy(a, b, n, cus * per_cu, 256, variant, st); }));
       LBM_HIP_TRY(hipGetLastError());
     }
+  const double rms = best_ms([&] { (void)launch_copy(a, b, n, cus * 32, 256, 0, st); });
+  double nms = 1e30;
+  const size_t n9 = n / kQ;
+  for (int nt : {0, 1}) nms = std::min(nms, best_ms([&] { (void)launch_copy9(a, b, n9, nt, st); }));
+  LBM_HIP_TRY(hipGetLastError());
   const double mms = best_ms([&] {
     (void)hipMemcpyAsync(b, a, n * sizeof(float), hipMemcpyDeviceToDevice, st);
   });
@@ -527,6 +533,8 @@ int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs) {
   const double total = 2.0 * static_cast<double>(n * sizeof(float));
   if (copy_gbs) *copy_gbs = total / (kms * 1e-3) / 1e9;
   if (memcpy_gbs) *memcpy_gbs = total / (mms * 1e-3) / 1e9;
+  if (reference_gbs) *reference_gbs = total / (rms * 1e-3) / 1e9;
+  if (nine_gbs) *nine_gbs = 2.0 * static_cast<double>(kQ * n9 * sizeof(float)) / (nms * 1e-3) / 1e9;
   return LBM_OK;
 }
 

@@ -58,6 +58,7 @@ struct SolverState {
   size_t staging_bytes = 0;
 };
 
-int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs);
+int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
+                 double* reference_gbs, double* nine_gbs);
 
 }  // namespace lbmhip

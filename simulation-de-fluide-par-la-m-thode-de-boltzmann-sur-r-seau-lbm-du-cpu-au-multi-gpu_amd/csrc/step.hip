@@ -124,6 +124,23 @@ __global__ void copy_kernel(const float* __restrict__ a, float* __restrict__ b, 
   }
 }
 
+template <bool kNt>
+__global__ void __launch_bounds__(256) copy9_kernel(const float* __restrict__ a, float* __restrict__ b,
+                                                    size_t n9) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n9) return;
+  float v[kQ];
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) v[k] = kNt ? __builtin_nontemporal_load(a + k * n9 + i) : a[k * n9 + i];
+#pragma unroll
+  for (int k = 0; k < kQ; ++k) {
+    if constexpr (kNt)
+      __builtin_nontemporal_store(v[k], b + k * n9 + i);
+    else
+      b[k * n9 + i] = v[k];
+  }
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(kReduceThreads)
@@ -209,6 +226,15 @@ template hipError_t launch_reduce<float>(const GatherParams<float>&, double*, do
                                          hipStream_t);
 template hipError_t launch_reduce<double>(const GatherParams<double>&, double*, double*,
                                           hipStream_t);
+
+hipError_t launch_copy9(const float* a, float* b, size_t n9, int nt, hipStream_t stream) {
+  const unsigned grid = static_cast<unsigned>((n9 + 255) / 256);
+  if (nt)
+    copy9_kernel<true><<<grid, 256, 0, stream>>>(a, b, n9);
+  else
+    copy9_kernel<false><<<grid, 256, 0, stream>>>(a, b, n9);
+  return hipGetLastError();
+}
 
 hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block, int variant,
                        hipStream_t stream) {
