@@ -190,10 +190,11 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
                        double* best_mlups);
 
 /* ---- discrete adjoint ---------------------------------------------------------------------- */
-/* No reference counterpart. Float forward of T steps with a per-cell force field and tau,
- * rest initial state; objective J = mean u_x (with the half-force term) over a probe window,
- * averaged over the last `avg_steps` states. Reverse sweep by the hand-written adjoint step
- * with binomial checkpointing (at most `snapshots` stored states). */
+/* No reference counterpart. Float or double forward of T steps with a per-cell force field and
+ * tau, rest initial state; objective J = mean u_x (with the half-force term) over a probe
+ * window, averaged over the last `avg_steps` states. Reverse sweep by the hand-written adjoint
+ * step with checkpointing in at most `snapshots` stored states (slot 0 holds the initial
+ * state). */
 typedef struct lbm_adjoint lbm_adjoint;
 int lbm_adjoint_create(int nx, int ny, double tau, const uint8_t* solid, int steps,
                        int probe_x0, int probe_y0, int probe_w, int probe_h, int avg_steps,
@@ -203,12 +204,34 @@ int lbm_adjoint_set_force_field(lbm_adjoint* a, const void* fxy);
 int lbm_adjoint_set_tau(lbm_adjoint* a, double tau);
 /* Forward only: J. */
 int lbm_adjoint_forward(lbm_adjoint* a, double* J, double* seconds);
-/* J and its gradient: dJ/dF (2N reals, planar) and dJ/dtau. */
+/* J and its gradient: dJ/dF (2N reals, planar) and dJ/dtau. `forward_steps`: forward steps of
+ * the whole gradient sweep, the first pass included. */
 int lbm_adjoint_gradient(lbm_adjoint* a, double* J, void* dJ_dforce, double* dJ_dtau,
                          double* seconds, long long* forward_steps);
 
-/* Host only (no device work): forward steps the checkpointing schedule recomputes to reverse
- * `states` states with `slots` stored states (slot 0 holds the initial state). */
+/* Checkpointing schedules (every one gives the same gradient bits):
+ *   LBM_SCHEDULE_BINOMIAL   revolve: the fewest forward steps for the snapshot budget; the
+ *                           forward steps run one after another.
+ *   LBM_SCHEDULE_PIPELINED  multi-level checkpoints; each reverse step shares its launch with
+ *                           one recomputation step per level (about T launches for the reverse
+ *                           sweep), within the snapshot budget.
+ *   LBM_SCHEDULE_AUTO       every state stored when the budget holds them all (no
+ *                           recomputation), else pipelined when it fits the budget, else
+ *                           binomial (the default). */
+#define LBM_SCHEDULE_AUTO 0
+#define LBM_SCHEDULE_BINOMIAL 1
+#define LBM_SCHEDULE_PIPELINED 2
+int lbm_adjoint_set_schedule(lbm_adjoint* a, int kind);
+/* The schedule in use: kind (BINOMIAL or PIPELINED), stored states, levels (0 for binomial),
+ * forward steps and launches of the gradient sweep. */
+int lbm_adjoint_schedule_info(lbm_adjoint* a, int* kind, int* slots, int* levels,
+                              long long* forward_steps, long long* launches);
+
+/* Host only (no device work): the schedule for `states` = T + 1 states within `slots` stored
+ * states (built and replay-checked); any output may be null. */
+int lbm_checkpoint_schedule(int states, int slots, int kind, int* slots_used, int* levels,
+                            long long* forward_steps, long long* launches);
+/* Host only: forward steps of the binomial schedule (first pass included). */
 int lbm_checkpoint_forward_steps(int states, int slots, long long* steps);
 
 /* ---- HBM copy ceiling (the counterpart of bench/gpu_bandwidth.cu) ---------------------------- */

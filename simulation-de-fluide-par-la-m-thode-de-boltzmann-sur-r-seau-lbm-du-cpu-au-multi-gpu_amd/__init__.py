@@ -121,6 +121,9 @@ def _declare(L):
         "lbm_adjoint_forward": ([vp, dp, dp], i),
         "lbm_adjoint_gradient": ([vp, dp, vp, dp, dp, llp], i),
         "lbm_checkpoint_forward_steps": ([i, i, llp], i),
+        "lbm_adjoint_set_schedule": ([vp, i], i),
+        "lbm_adjoint_schedule_info": ([vp, ip, ip, ip, llp, llp], i),
+        "lbm_checkpoint_schedule": ([i, i, i, ip, ip, llp, llp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -559,19 +562,37 @@ class Batch:
 # ---- discrete adjoint ------------------------------------------------------------------------
 
 def checkpoint_forward_steps(states: int, slots: int) -> int:
-    """Forward steps recomputed by the checkpointing schedule (host computation only)."""
+    """Forward steps of the binomial checkpointing schedule, first pass included (host
+    computation only)."""
     n = C.c_longlong()
     check(lib().lbm_checkpoint_forward_steps(int(states), int(slots), C.byref(n)))
     return n.value
 
 
+SCHEDULES = {"auto": 0, "binomial": 1, "pipelined": 2}
+_SCHEDULE_NAMES = {v: k for k, v in SCHEDULES.items()}
+
+
+def checkpoint_schedule(states: int, slots: int, kind: str = "auto") -> dict:
+    """The checkpointing schedule for `states` = T + 1 states within `slots` stored states
+    (built and replay-checked on the host): kind, stored states used, levels, forward steps and
+    launches of the gradient sweep."""
+    used, levels = C.c_int(), C.c_int()
+    fsteps, launches = C.c_longlong(), C.c_longlong()
+    check(lib().lbm_checkpoint_schedule(int(states), int(slots), SCHEDULES[kind], C.byref(used),
+                                        C.byref(levels), C.byref(fsteps), C.byref(launches)))
+    return {"kind": "pipelined" if levels.value else "binomial", "slots": used.value,
+            "levels": levels.value, "forward_steps": fsteps.value, "launches": launches.value}
+
+
 class Adjoint:
     """Gradient of J = mean u_x over a probe window, averaged over the last `avg_steps` states of
     a `steps`-step run from rest, w.r.t. a per-cell force field and tau (discrete adjoint with
-    binomial checkpointing in `snapshots` stored states)."""
+    checkpointing in at most `snapshots` stored states; `schedule`: "auto", "binomial" or
+    "pipelined")."""
 
     def __init__(self, nx, ny, tau, solid, steps, probe, avg_steps, snapshots,
-                 precision="float"):
+                 precision="float", schedule="auto"):
         self.precision = precision_code(precision)
         self.dtype = real_dtype(self.precision)
         self.nx, self.ny = int(nx), int(ny)
@@ -584,6 +605,20 @@ class Adjoint:
                                        y0, w, h, int(avg_steps), int(snapshots), self.precision,
                                        C.byref(hd)))
         self._h = hd
+        if schedule != "auto":
+            self.set_schedule(schedule)
+
+    def set_schedule(self, kind: str):
+        check(lib().lbm_adjoint_set_schedule(self._h, SCHEDULES[kind]))
+
+    def schedule_info(self) -> dict:
+        kind, slots, levels = C.c_int(), C.c_int(), C.c_int()
+        fsteps, launches = C.c_longlong(), C.c_longlong()
+        check(lib().lbm_adjoint_schedule_info(self._h, C.byref(kind), C.byref(slots),
+                                              C.byref(levels), C.byref(fsteps),
+                                              C.byref(launches)))
+        return {"kind": _SCHEDULE_NAMES[kind.value], "slots": slots.value, "levels": levels.value,
+                "forward_steps": fsteps.value, "launches": launches.value}
 
     def close(self):
         h = getattr(self, "_h", None)

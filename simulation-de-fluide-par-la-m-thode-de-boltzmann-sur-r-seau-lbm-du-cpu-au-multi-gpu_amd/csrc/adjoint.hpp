@@ -1,4 +1,11 @@
-// Parameters and launchers of the discrete-adjoint kernels (adjoint.hip).
+// Parameters and launcher of the discrete-adjoint kernels (adjoint.hip).
+//
+// One launch runs up to one reverse step and up to kMaxChains forward steps on disjoint
+// buffers (a checkpointing schedule packs a reverse step with the recomputations that do not
+// depend on it). Every step is one thread per cell with a single memory round trip: the cells
+// that can need a bounce-back operand (solid cells, links to solids) lie inside a few "dirty
+// boxes" passed as kernel arguments, and only there the cell's own opposite populations are
+// loaded, together with everything else, before the link code is looked at.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,39 +17,65 @@
 
 namespace lbmhip {
 
+constexpr int kMaxBoxes = 8;
+constexpr int kAdjMaxFwd = 3;
+
+// Half-open boxes [x0, x1) x [y0, y1) covering every cell whose link code is not 0.
+struct DirtyBoxes {
+  int n;
+  int x0[kMaxBoxes], x1[kMaxBoxes], y0[kMaxBoxes], y1[kMaxBoxes];
+};
+
 template <class T>
-struct AdjointParams {
-  const T* R;          // forward state R_t (pull layout)
-  const T* mu_in;      // mu_{t+1}, or null for the last state (no later step)
-  T* mu_out;           // mu_t
+struct AdjGeom {
   const uint16_t* links;
-  const T* force;      // per-cell force field (fx plane, fy plane)
-  T* dforce;           // dJ/dF accumulators (2N)
-  double* partials;    // two slots per block: dJ/dtau, J
+  const T* force;  // per-cell force field: fx plane, fy plane (N each)
   size_t N;
   int nx, ny;
   T omega, fp;
-  int direct;          // 1 if state t lies in the averaging window
-  int px0, py0, pw, ph;
-  T k;                 // 1 / (window steps * probe cells)
-  double kd;           // the same in FP64, for the J partials
+  int px0, py0, pw, ph;  // probe window
+  T k;                   // 1 / (window states * probe cells)
+  double kd;             // the same in FP64 (J accumulation)
+  DirtyBoxes boxes;
 };
 
-// Block size of the reverse kernels, and tile width of the forward part of a fused launch.
+// One forward step src -> dst. jacc: FP64 accumulators of J, one per probe cell (row-major over
+// the window), when the state src lies in the averaging window; null otherwise. With dst null
+// the step only accumulates J (the last state of a run has no successor).
+template <class T>
+struct AdjFwd {
+  const T* src;
+  T* dst;
+  double* jacc;
+};
+
+// One reverse step: mu_t from mu_{t+1} and R_t, plus the gradient accumulators.
+template <class T>
+struct AdjRev {
+  const T* R;       // forward state R_t
+  const T* mu_in;   // mu_{t+1}, or null for the last state (no later step)
+  T* mu_out;        // mu_t
+  T* dforce;        // dJ/dF accumulators (2N)
+  double* tacc;     // dJ/dtau accumulators, one FP64 per cell
+  double* jacc;     // J accumulators per probe cell when R_t lies in the window, else null
+  int direct;       // 1 if R_t lies in the averaging window (direct term of J)
+};
+
+// Cells per block of the adjoint kernels (one thread per cell).
 constexpr int kAdjointThreads = 256;
 
 __host__ __device__ inline unsigned adjoint_blocks(size_t N) {
   return static_cast<unsigned>((N + kAdjointThreads - 1) / kAdjointThreads);
 }
 
+// A launch: the reverse step `rev` when has_rev, then nfwd forward steps fwd[0..nfwd).
 template <class T>
-hipError_t launch_adjoint_step(const AdjointParams<T>& p, hipStream_t stream);
-template <class T>
-hipError_t launch_adjoint_fused(const AdjointParams<T>& a, const StepParams<T>& f,
-                                hipStream_t stream);
-template <class T>
-hipError_t launch_probe(const GatherParams<T>& p, int px0, int py0, int pw, int ph, double k,
-                        double* acc, hipStream_t stream);
+hipError_t launch_adjoint(const AdjGeom<T>& g, const AdjRev<T>& rev, bool has_rev,
+                          const AdjFwd<T>* fwd, int nfwd, hipStream_t stream);
+
+// out[0] = sum of a[0..n) (fixed order: deterministic), on one block.
+hipError_t launch_sum_f64(const double* a, size_t n, double* out, hipStream_t stream);
+
 template <class T>
 hipError_t launch_fill(T* a, size_t n, T v, hipStream_t stream);
 

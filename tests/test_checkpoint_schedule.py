@@ -37,3 +37,49 @@ def test_degenerate_budgets(lbm):
 def test_differentiation_instance_budget(lbm):
     # T = 2000 steps -> 2001 states, 45 ~ sqrt(T) slots: r = 3
     assert lbm.checkpoint_forward_steps(2001, 45) == 3 * 2001 - comb(48, 2)
+
+
+# ---- pipelined multi-level schedule -------------------------------------------------------------
+# Every launch of its reverse sweep holds one reverse step (plus one recomputation step per
+# level), so the whole gradient is T forward launches then T + 1 reverse launches when no launch
+# stalls. The library replays every schedule it builds on buffer contents (each step must read
+# the state it expects, reverse steps must run T, ..., 0) and refuses one that fails.
+
+@pytest.mark.parametrize("states", [2, 3, 10, 81, 500, 2001])
+@pytest.mark.parametrize("slots", [6, 12, 20, 45, 91])
+def test_pipelined_within_budget_or_binomial(lbm, states, slots):
+    s = lbm.checkpoint_schedule(states, slots, "auto")
+    assert s["slots"] <= max(slots, 1)
+    if s["kind"] == "pipelined":
+        T = states - 1
+        assert 2 <= s["levels"] <= 4
+        assert s["launches"] >= 2 * T + 1
+        # one recomputation step per level and reverse step at most, plus the first pass
+        assert T <= s["forward_steps"] <= T + (s["levels"] - 1) * states
+    else:
+        assert s["forward_steps"] == griewank(states, min(slots, states))
+
+
+def test_differentiation_instance_pipelined_at_sqrt_T(lbm):
+    # T = 2000, at most 45 ~ sqrt(T) stored states: three levels, no stalled launch
+    s = lbm.checkpoint_schedule(2001, 45, "pipelined")
+    assert s["kind"] == "pipelined" and s["levels"] == 3 and s["slots"] <= 45
+    assert s["launches"] == 2 * 2000 + 1
+    b = lbm.checkpoint_schedule(2001, 45, "binomial")
+    assert b["forward_steps"] == 3 * 2001 - comb(48, 2)
+    assert b["launches"] > s["launches"] + 2000
+
+
+def test_pipelined_levels_follow_the_budget(lbm):
+    # fewer stored states need more levels (more recomputation chains per launch)
+    two = lbm.checkpoint_schedule(2001, 91, "pipelined")
+    three = lbm.checkpoint_schedule(2001, 45, "pipelined")
+    four = lbm.checkpoint_schedule(2001, 30, "pipelined")
+    assert (two["levels"], three["levels"], four["levels"]) == (2, 3, 4)
+    assert two["forward_steps"] < three["forward_steps"] < four["forward_steps"]
+
+
+def test_pipelined_rejects_budgets_too_small(lbm):
+    with pytest.raises(ValueError, match="pipelined"):
+        lbm.checkpoint_schedule(2001, 6, "pipelined")
+    assert lbm.checkpoint_schedule(2001, 6, "auto")["kind"] == "binomial"

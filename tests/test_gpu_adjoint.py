@@ -70,28 +70,67 @@ def test_small_case_matches_finite_differences(lbm, prec, tol):
 
 @pytest.mark.parametrize("prec", ("float", "double"))
 def test_checkpoint_schedules_give_identical_gradients(lbm, prec):
-    """Recomputed states are bit-identical to stored ones, so every snapshot budget gives the
-    same bits; the number of recomputed steps follows the binomial schedule."""
+    """Recomputed states are bit-identical to stored ones, so every schedule and snapshot budget
+    gives the same bits: binomial schedules (their recomputation count follows the binomial
+    optimum) and pipelined ones (one recomputation step per level inside each reverse launch)."""
     nx, ny, steps, avg = 64, 24, 37, 9
     solid = disc_channel(lbm, nx, ny, 30.0, 12.0, 3.0)
     F = force_field(nx, ny, 3)
     ref = None
     counts = {}
-    for snaps in (1, 2, 3, 6, 38, 100):
-        a = lbm.Adjoint(nx, ny, 0.8, solid, steps, (44, 6, 6, 10), avg, snaps, prec)
+    runs = [("binomial", n) for n in (1, 2, 3, 6, 38, 100)] + \
+           [("pipelined", n) for n in (9, 10, 14, 38)]
+    for kind, snaps in runs:
+        a = lbm.Adjoint(nx, ny, 0.8, solid, steps, (44, 6, 6, 10), avg, snaps, prec, kind)
+        info = a.schedule_info()
+        assert info["kind"] == kind and info["slots"] <= snaps
         a.set_force_field(F)
         J, dF, dtau, _, fsteps = a.gradient()
-        counts[snaps] = fsteps
+        a.close()
+        if kind == "binomial":
+            counts[snaps] = fsteps
+        else:
+            assert info["launches"] == 2 * steps + 1  # every reverse launch carries the chains
         if ref is None:
             ref = (J, dF.copy(), dtau)
         else:
-            assert np.array_equal(dF, ref[1]), snaps
-            assert dtau == ref[2] or abs(dtau - ref[2]) <= 1e-12 * abs(ref[2])
-            assert abs(J - ref[0]) <= 1e-13 * abs(ref[0])
+            assert np.array_equal(dF, ref[1]), (kind, snaps)
+            assert dtau == ref[2], (kind, snaps)
+            assert J == ref[0], (kind, snaps)
     n = steps + 1
     assert counts[1] == n * (n - 1) // 2   # no free slot: recompute from R_0 every time
     assert counts[38] == steps and counts[100] == steps  # every state stored once
     assert counts[1] > counts[2] > counts[3] > counts[6] > counts[38]
+
+
+@pytest.mark.parametrize("prec", ("float", "double"))
+def test_forward_matches_the_plain_step(lbm, prec):
+    """The adjoint's forward steps are the plain step's arithmetic (collide() on the same pulled
+    values): J from the adjoint's forward run equals J from the plain solver (bit-identical to
+    the reference's v0) stepped with the same force field, read back state by state."""
+    nx, ny, steps, avg = 48, 20, 30, 7
+    probe = (30, 5, 6, 9)
+    solid = disc_channel(lbm, nx, ny, 14.0, 10.0, 3.5)
+    F = force_field(nx, ny, 9, gx=3e-5, sigma=1e-6)
+    a = lbm.Adjoint(nx, ny, 0.75, solid, steps, probe, avg, 4, prec)
+    a.set_force_field(F)
+    J, _ = a.forward()
+    Jg = a.gradient()[0]
+    s = lbm.Solver(nx, ny, 0.75, 0.0, 0.0, solid, prec)
+    s.set_force_field(F)
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    x0, y0, w, h = probe
+    acc = 0.0
+    for t in range(steps + 1):
+        if t > steps - avg:
+            ux = s.macroscopic()[1].reshape(ny, nx)[y0:y0 + h, x0:x0 + w].astype(np.float64)
+            acc += float(np.sum(ux))
+        if t < steps:
+            s.step()
+    s.close()
+    Jp = acc / (avg * w * h)
+    assert abs(J - Jp) <= 1e-12 * abs(Jp), (J, Jp)
+    assert abs(Jg - Jp) <= 1e-12 * abs(Jp), (Jg, Jp)
 
 
 def test_baseline_instance_512x128_T2000(lbm):
@@ -106,11 +145,14 @@ def test_baseline_instance_512x128_T2000(lbm):
     F = force_field(nx, ny, 2024)
     tau = 0.7
     a = lbm.Adjoint(nx, ny, tau, solid, steps, probe, avg, 45, "float")
+    info = a.schedule_info()
+    assert info["kind"] == "pipelined" and info["slots"] <= 45
     a.set_force_field(F)
     J, dF, dtau, t_grad, fsteps = a.gradient()
     Jf, t_fwd = a.forward()
+    assert abs(J - Jf) <= 1e-12 * abs(Jf)
     print(f"J={J:.6e} forward {t_fwd * 1e3:.2f} ms, gradient {t_grad * 1e3:.2f} ms "
-          f"(x{t_grad / t_fwd:.2f}), recomputed forward steps {fsteps}")
+          f"(x{t_grad / t_fwd:.2f}), forward steps {fsteps}, schedule {info}")
     rng = np.random.default_rng(77)
     fluid = ~solid.reshape(ny, nx).astype(bool)
     worst = 0.0
