@@ -54,130 +54,197 @@ __device__ __forceinline__ Nbr neighbours(int x, int y, int nx, int ny) {
                   static_cast<unsigned>(x == nx - 1 ? 0 : x + 1) * B}};
 }
 
-// Forward step of one cell: the same values and the same collide() as step_kernel (the state is
-// bit-identical); optionally accumulates k * ux of the pulled state for J.
+// Makes the compiler hold the value of x at this point (an empty asm with x as an in/out
+// register operand): loads issued before it cannot be moved below it. The kernels pin every
+// loaded value after their load phase, so all loads of a launch are in flight before the first
+// wait, instead of the population loads being sunk below the solid-cell test.
 template <class T>
-__device__ __forceinline__ void forward_cell(const AdjGeom<T>& G, const T* src, T* dst,
-                                             double* jacc, unsigned c, int x, int y, bool dirty) {
-  const unsigned stride = static_cast<unsigned>(G.N * sizeof(T));
-  const Nbr nb = neighbours<T>(x, y, G.nx, G.ny);
-  const Rsrc rs = make_rsrc(src);
-  T v[kQ], own[kQ];
+__device__ __forceinline__ void pin(T& x) {
+  asm volatile("" : "+v"(x));
+}
+template <class T, int n>
+__device__ __forceinline__ void pin(T (&x)[n], int from = 0) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) v[i] = Stored<T>::load(rs, nb.pull(i), i * stride);
-  uint32_t link = 0;
-  if (dirty) {
-    link = G.links[c];
-#pragma unroll
-    for (int i = 1; i < kQ; ++i) own[i] = Stored<T>::load(rs, nb.self(), i * stride);
-  }
-  const T fx = G.force[c], fy = G.force[G.N + c];
-  const bool probe = jacc && x >= G.px0 && x < G.px0 + G.pw && y >= G.py0 && y < G.py0 + G.ph;
-  const unsigned pj = static_cast<unsigned>((y - G.py0) * G.pw + (x - G.px0));
-  double j0 = 0.0;
-  if (probe) j0 = jacc[pj];
-  if (link & kLinkSolid) return;
-  if (dirty)
-#pragma unroll
-    for (int i = 1; i < kQ; ++i)
-      if (link & (1u << i)) v[i] = own[kOpp[i]];
-  if (probe) {
-    T r, u, w;
-    moments(v, fx, fy, r, u, w);
-    jacc[pj] = j0 + G.kd * static_cast<double>(u);
-  }
-  if (!dst) return;
-  collide(v, fx, fy, G.omega, G.fp);
-  const Rsrc rd = make_rsrc(dst);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) Stored<T>::store(rd, nb.self(), i * stride, v[i]);
+  for (int i = from; i < n; ++i) pin(x[i]);
 }
 
-// Reverse step of one cell.
+// Values of one cell shared by every role of a launch (the same cell, mask and force field).
 template <class T>
-__device__ __forceinline__ void reverse_cell(const AdjGeom<T>& G, const AdjRev<T>& p, unsigned c,
-                                             int x, int y, bool dirty) {
-  const unsigned stride = static_cast<unsigned>(G.N * sizeof(T));
-  const Nbr nb = neighbours<T>(x, y, G.nx, G.ny);
-  const Rsrc rr = make_rsrc(p.R);
-  const Rsrc rm = make_rsrc(p.mu_in ? p.mu_in : p.R);  // never read when mu_in is null
-  const bool have_mu = p.mu_in != nullptr;
-  // every load of the cell is issued before any is used: one memory round trip
-  T g[kQ], lam[kQ], ownR[kQ], ownM[kQ];
+struct CellIn {
+  uint32_t link;
+  T fx, fy;
+};
+
+// Forward step of one cell, split into a load phase and a compute/store phase so that a launch
+// can issue the loads of all its roles before it waits for any of them. The values and the
+// collide() are those of step_kernel: the state is bit-identical.
+template <class T>
+struct FwdIn {
+  T v[kQ], own[kQ];
+  double j0;
+  __device__ __forceinline__ void pin_all() {
+    pin(v);
+    pin(own, 1);
+    pin(j0);
+  }
+};
+
+template <class T>
+__device__ __forceinline__ void forward_load(const T* src, double* jacc, const Nbr& nb,
+                                             unsigned stride, bool dirty, bool probe, unsigned pj,
+                                             FwdIn<T>& in) {
+  const Rsrc rs = make_rsrc(src);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) g[i] = Stored<T>::load(rr, nb.pull(i), i * stride);
+  for (int i = 0; i < kQ; ++i) in.v[i] = Stored<T>::load(rs, nb.pull(i), i * stride);
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) in.own[i] = T(0);
+  if (dirty)  // wave-uniform
+#pragma unroll
+    for (int i = 1; i < kQ; ++i) in.own[i] = Stored<T>::load(rs, nb.self(), i * stride);
+  in.j0 = 0.0;
+  if (jacc) in.j0 = jacc[probe ? pj : 0u];  // uniform branch, in-range index for every lane
+}
+
+template <class T>
+__device__ __forceinline__ void forward_finish(const AdjGeom<T>& G, T* dst, double* jacc,
+                                               const Nbr& nb, unsigned stride, bool dirty,
+                                               bool probe, unsigned pj, const CellIn<T>& cell,
+                                               FwdIn<T>& in) {
+#pragma unroll
+  for (int i = 1; i < kQ; ++i)
+    if (cell.link & (1u << i)) in.v[i] = in.own[kOpp[i]];
+  if (probe && jacc) {
+    T r, u, w;
+    moments(in.v, cell.fx, cell.fy, r, u, w);
+    jacc[pj] = in.j0 + G.kd * static_cast<double>(u);
+  }
+  if (!dst) return;
+  collide(in.v, cell.fx, cell.fy, G.omega, G.fp);
+  const Rsrc rd = make_rsrc(dst);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) Stored<T>::store(rd, nb.self(), i * stride, in.v[i]);
+}
+
+// Reverse step of one cell, in the same two phases.
+template <class T>
+struct RevIn {
+  T g[kQ], lam[kQ], ownR[kQ], ownM[kQ];
+  T dfx0, dfy0;
+  double t0, j0;
+  __device__ __forceinline__ void pin_all() {
+    pin(g);
+    pin(lam);
+    pin(ownR, 1);
+    pin(ownM, 1);
+    pin(dfx0);
+    pin(dfy0);
+    pin(t0);
+    pin(j0);
+  }
+};
+
+template <class T>
+__device__ __forceinline__ void reverse_load(const AdjGeom<T>& G, const AdjRev<T>& p, unsigned c,
+                                             const Nbr& nb, unsigned stride, bool dirty,
+                                             bool probe, unsigned pj, RevIn<T>& in) {
+  const Rsrc rr = make_rsrc(p.R);
+  const bool have_mu = p.mu_in != nullptr;
+  const Rsrc rm = make_rsrc(have_mu ? p.mu_in : p.R);  // never read when mu_in is null
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) in.g[i] = Stored<T>::load(rr, nb.pull(i), i * stride);
   if (have_mu) {
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) lam[i] = Stored<T>::load(rm, nb.push(i), i * stride);
+    for (int i = 0; i < kQ; ++i) in.lam[i] = Stored<T>::load(rm, nb.push(i), i * stride);
   } else {
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) lam[i] = T(0);
+    for (int i = 0; i < kQ; ++i) in.lam[i] = T(0);
   }
-  uint32_t link = 0;
-  if (dirty) {
-    link = G.links[c];
 #pragma unroll
-    for (int i = 1; i < kQ; ++i) ownR[i] = Stored<T>::load(rr, nb.self(), i * stride);
+  for (int i = 1; i < kQ; ++i) in.ownR[i] = in.ownM[i] = T(0);
+  if (dirty) {  // wave-uniform
+#pragma unroll
+    for (int i = 1; i < kQ; ++i) in.ownR[i] = Stored<T>::load(rr, nb.self(), i * stride);
     if (have_mu)
 #pragma unroll
-      for (int i = 1; i < kQ; ++i) ownM[i] = Stored<T>::load(rm, nb.self(), i * stride);
+      for (int i = 1; i < kQ; ++i) in.ownM[i] = Stored<T>::load(rm, nb.self(), i * stride);
   }
-  const T fx = G.force[c], fy = G.force[G.N + c];
-  const T dfx0 = p.dforce[c], dfy0 = p.dforce[G.N + c];
-  const double t0 = p.tacc[c];
-  const bool probe =
-      p.direct && x >= G.px0 && x < G.px0 + G.pw && y >= G.py0 && y < G.py0 + G.ph;
-  const unsigned pj = static_cast<unsigned>((y - G.py0) * G.pw + (x - G.px0));
-  double j0 = 0.0;
-  if (probe && p.jacc) j0 = p.jacc[pj];
-  if (link & kLinkSolid) return;
-  if (dirty) {
+  in.dfx0 = p.dforce[c];
+  in.dfy0 = p.dforce[G.N + c];
+  in.t0 = p.tacc[c];
+  in.j0 = 0.0;
+  if (p.jacc) in.j0 = p.jacc[probe ? pj : 0u];  // uniform branch, in-range index for every lane
+}
+
+template <class T>
+__device__ __forceinline__ void reverse_finish(const AdjGeom<T>& G, const AdjRev<T>& p, unsigned c,
+                                               const Nbr& nb, unsigned stride, bool dirty,
+                                               bool probe, unsigned pj, const CellIn<T>& cell,
+                                               RevIn<T>& in) {
+#pragma clang fp contract(fast)  // the reverse step has no bitwise counterpart
+  const T fx = cell.fx, fy = cell.fy;
+#pragma unroll
+  for (int i = 1; i < kQ; ++i)
+    if (cell.link & (1u << i)) in.g[i] = in.ownR[kOpp[i]];
+  if (p.mu_in)
 #pragma unroll
     for (int i = 1; i < kQ; ++i)
-      if (link & (1u << i)) g[i] = ownR[kOpp[i]];
-    if (have_mu)
-#pragma unroll
-      for (int i = 1; i < kQ; ++i)
-        if (link & (1u << kOpp[i])) lam[i] = ownM[kOpp[i]];
-  }
-
-  T drho = T(0), mx = T(0), my = T(0);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    drho += g[i];
-    mx += T(kCx[i]) * g[i];
-    my += T(kCy[i]) * g[i];
-  }
+      if (cell.link & (1u << kOpp[i])) in.lam[i] = in.ownM[kOpp[i]];
+  const T(&g)[kQ] = in.g;
+  const T(&lam)[kQ] = in.lam;
+  // The transposed Jacobian in moment form.
+  // With a_i = lambda_i w_i and its moments S0 = sum a, Sx = sum cx a, Sy = sum cy a,
+  // Sxx = sum cx^2 a, Syy = sum cy^2 a, Sxy = sum cx cy a (cx, cy in {-1, 0, 1}), every
+  // direction sum of the BGK + Guo Jacobian is a polynomial in them:
+  //   sum a P        = 3 (ux Sx + uy Sy) + 4.5 (ux^2 Sxx + 2 ux uy Sxy + uy^2 Syy) - 1.5 |u|^2 S0
+  //   sum a cu cx    = ux Sxx + uy Sxy        sum a cu cy = ux Sxy + uy Syy
+  //   sum a cx cf    = fx Sxx + fy Sxy        sum a cy cf = fx Sxy + fy Syy
+  //   sum a cu cf    = ux fx Sxx + (ux fy + uy fx) Sxy + uy fy Syy
+  const T drho = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+  const T mx = ((g[1] - g[3]) + (g[5] - g[6])) + (g[8] - g[7]);
+  const T my = ((g[2] - g[4]) + (g[5] + g[6])) - (g[7] + g[8]);
   const T rho = T(1) + drho;
-  const T ux = (mx + T(0.5) * fx) / rho, uy = (my + T(0.5) * fy) / rho;
+  const T ir = T(1) / rho;
+  const T ux = (mx + T(0.5) * fx) * ir, uy = (my + T(0.5) * fy) * ir;
   const T usq = ux * ux + uy * uy;
   const T om = G.omega, fp = G.fp;
-
-  T A_drho = T(0), A_ux = T(0), A_uy = T(0), dfx = T(0), dfy = T(0), dom = T(0);
+  const T w0 = weight<T>(0), w1 = weight<T>(1), w5 = weight<T>(5);
+  const T a0 = w0 * lam[0];
+  const T a1 = w1 * lam[1], a2 = w1 * lam[2], a3 = w1 * lam[3], a4 = w1 * lam[4];
+  const T a5 = w5 * lam[5], a6 = w5 * lam[6], a7 = w5 * lam[7], a8 = w5 * lam[8];
+  const T dsum = (a5 + a6) + (a7 + a8);  // diagonals
+  const T S0 = a0 + ((a1 + a2) + (a3 + a4)) + dsum;
+  const T Sx = (a1 - a3) + ((a5 - a6) - (a7 - a8));
+  const T Sy = (a2 - a4) + ((a5 + a6) - (a7 + a8));
+  const T Sxx = (a1 + a3) + dsum;
+  const T Syy = (a2 + a4) + dsum;
+  const T Sxy = (a5 - a6) + (a7 - a8);
+  T Slg = T(0);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const T cx = T(kCx[i]), cy = T(kCy[i]), w = weight<T>(i);
-    const T cu = cx * ux + cy * uy;
-    const T P = T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq;
-    const T geq = w * (drho + rho * P);
-    const T cf = cx * fx + cy * fy;
-    const T src = w * (T(3) * ((cx - ux) * fx + (cy - uy) * fy) + T(9) * cu * cf);
-    const T l = lam[i];
-    A_drho += l * (om * w * (T(1) + P));
-    A_ux += l * (om * w * rho * (T(3) * cx + T(9) * cu * cx - T(3) * ux) +
-                 fp * w * (T(-3) * fx + T(9) * cx * cf));
-    A_uy += l * (om * w * rho * (T(3) * cy + T(9) * cu * cy - T(3) * uy) +
-                 fp * w * (T(-3) * fy + T(9) * cy * cf));
-    dfx += l * (fp * w * (T(3) * (cx - ux) + T(9) * cu * cx));
-    dfy += l * (fp * w * (T(3) * (cy - uy) + T(9) * cu * cy));
-    dom += l * (geq - g[i] - T(0.5) * src);
-  }
-  // direct term of J: k * ux at probe cells, states inside the averaging window
+  for (int i = 0; i < kQ; ++i) Slg += lam[i] * g[i];
+  const T Acu = ux * Sx + uy * Sy;
+  const T AP = T(3) * Acu + T(4.5) * (ux * ux * Sxx + T(2) * ux * uy * Sxy + uy * uy * Syy) -
+               T(1.5) * usq * S0;
+  const T Ax = ux * Sxx + uy * Sxy, Ay = ux * Sxy + uy * Syy;
+  const T Fx = fx * Sxx + fy * Sxy, Fy = fx * Sxy + fy * Syy;
+  const T A_drho = om * (S0 + AP);
+  T A_ux = om * rho * (T(3) * Sx + T(9) * Ax - T(3) * ux * S0) + fp * (T(-3) * fx * S0 + T(9) * Fx);
+  const T A_uy = om * rho * (T(3) * Sy + T(9) * Ay - T(3) * uy * S0) + fp * (T(-3) * fy * S0 + T(9) * Fy);
+  const T dfx = fp * (T(3) * (Sx - ux * S0) + T(9) * Ax);
+  const T dfy = fp * (T(3) * (Sy - uy * S0) + T(9) * Ay);
+  const T Acf = ux * fx * Sxx + (ux * fy + uy * fx) * Sxy + uy * fy * Syy;
+  const T Lsrc = T(3) * ((fx * Sx + fy * Sy) - (ux * fx + uy * fy) * S0) + T(9) * Acf;
+  const T dom = (drho * S0 + rho * AP) - Slg - T(0.5) * Lsrc;
+  // direct term of J: k * ux at probe cells, states inside the averaging window; J itself
+  // accumulates the forward's own ux (moments() in the step's operation order, no contraction),
+  // so the gradient sweep reports the same J as the forward run
   if (probe) {
     A_ux += G.k;
-    if (p.jacc) p.jacc[pj] = j0 + G.kd * static_cast<double>(ux);
+    if (p.jacc) {
+      T r, u, w;
+      moments(g, fx, fy, r, u, w);
+      p.jacc[pj] = in.j0 + G.kd * static_cast<double>(u);
+    }
   }
-  const T ir = T(1) / rho;
   const T A_mx = A_ux * ir, A_my = A_uy * ir;
   const T A_rho = A_drho - (A_ux * ux + A_uy * uy) * ir;
   const Rsrc ro = make_rsrc(p.mu_out);
@@ -185,35 +252,60 @@ __device__ __forceinline__ void reverse_cell(const AdjGeom<T>& G, const AdjRev<T
   for (int i = 0; i < kQ; ++i)
     Stored<T>::store(ro, nb.self(), i * stride,
                      (T(1) - om) * lam[i] + A_rho + T(kCx[i]) * A_mx + T(kCy[i]) * A_my);
-  p.dforce[c] = dfx0 + (dfx + A_ux * T(0.5) * ir);
-  p.dforce[G.N + c] = dfy0 + (dfy + A_uy * T(0.5) * ir);
+  p.dforce[c] = in.dfx0 + (dfx + A_ux * T(0.5) * ir);
+  p.dforce[G.N + c] = in.dfy0 + (dfy + A_uy * T(0.5) * ir);
   const T dtau = -(om * om) * dom;  // d omega / d tau = -omega^2
-  p.tacc[c] = t0 + static_cast<double>(dtau);
+  p.tacc[c] = in.t0 + static_cast<double>(dtau);
 }
 
-// Blocks [0, nb) run the reverse step (when has_rev), the next nb blocks each forward step.
-template <class T>
-__global__ void __launch_bounds__(kAdjointThreads)
-    adjoint_kernel(const AdjGeom<T> G, const AdjRev<T> r, const AdjFwd<T> f0, const AdjFwd<T> f1,
-                   const AdjFwd<T> f2, int has_rev) {
+// Blocks [0, nb) run the reverse step (kRev), the next nb blocks each forward step: the roles
+// run in different waves, each with a single memory round trip.
+template <class T, int kRev>
+__global__ void __launch_bounds__(kAdjointThreads) adjoint_kernel(const AdjArgs<T> a) {
+  const AdjGeom<T>& G = a.g;
   const unsigned nb = adjoint_blocks(G.N);
-  unsigned role = blockIdx.x / nb;
+  const unsigned role = blockIdx.x / nb;
   const unsigned c = (blockIdx.x - role * nb) * kAdjointThreads + threadIdx.x;
   if (c >= G.N) return;
   const unsigned nx = static_cast<unsigned>(G.nx);
   const int y = static_cast<int>(c / nx), x = static_cast<int>(c - static_cast<unsigned>(y) * nx);
-  const bool dirty = in_boxes(G.boxes, x, y);
-  if (has_rev) {
-    if (role == 0) {
-      reverse_cell(G, r, c, x, y, dirty);
-      return;
-    }
-    --role;
+  // Waves with a cell in a dirty box load the link codes and the cells' own populations (for
+  // every lane: valid addresses, and a clean cell's link code is 0); the others skip them.
+  const bool dirty = __builtin_amdgcn_ballot_w64(in_boxes(G.boxes, x, y)) != 0;
+  const bool probe = (x >= G.px0) & (x < G.px0 + G.pw) & (y >= G.py0) & (y < G.py0 + G.ph);
+  const unsigned pj = static_cast<unsigned>((y - G.py0) * G.pw + (x - G.px0));
+  const unsigned stride = static_cast<unsigned>(G.N * sizeof(T));
+  const Nbr nb3 = neighbours<T>(x, y, G.nx, G.ny);
+
+  CellIn<T> cell;
+  cell.link = 0u;
+  if (dirty) cell.link = G.links[c];
+  cell.fx = G.force[c];
+  cell.fy = G.force[G.N + c];
+  if (kRev && role == 0) {
+    const bool pr = probe && a.rev.direct;
+    RevIn<T> in;
+    reverse_load(G, a.rev, c, nb3, stride, dirty, pr, pj, in);
+    pin(cell.link);
+    pin(cell.fx);
+    pin(cell.fy);
+    in.pin_all();
+    if (cell.link & kLinkSolid) return;
+    reverse_finish(G, a.rev, c, nb3, stride, dirty, pr, pj, cell, in);
+    return;
   }
-  const T* src = role == 0 ? f0.src : (role == 1 ? f1.src : f2.src);
-  T* dst = role == 0 ? f0.dst : (role == 1 ? f1.dst : f2.dst);
-  double* jacc = role == 0 ? f0.jacc : (role == 1 ? f1.jacc : f2.jacc);
-  forward_cell(G, src, dst, jacc, c, x, y, dirty);
+  const unsigned k = role - (kRev ? 1u : 0u);
+  const T* src = k == 0 ? a.fwd[0].src : (k == 1 ? a.fwd[1].src : a.fwd[2].src);
+  T* dst = k == 0 ? a.fwd[0].dst : (k == 1 ? a.fwd[1].dst : a.fwd[2].dst);
+  double* jacc = k == 0 ? a.fwd[0].jacc : (k == 1 ? a.fwd[1].jacc : a.fwd[2].jacc);
+  FwdIn<T> in;
+  forward_load(src, jacc, nb3, stride, dirty, probe, pj, in);
+  pin(cell.link);
+  pin(cell.fx);
+  pin(cell.fy);
+  in.pin_all();
+  if (cell.link & kLinkSolid) return;
+  forward_finish(G, dst, jacc, nb3, stride, dirty, probe, pj, cell, in);
 }
 
 __global__ void __launch_bounds__(kReduceThreads) sum_f64_kernel(const double* __restrict__ a,
@@ -241,10 +333,15 @@ hipError_t launch_adjoint(const AdjGeom<T>& g, const AdjRev<T>& rev, bool has_re
       static_cast<uint64_t>(kQ) * g.N * sizeof(T) >= (uint64_t{1} << 32) || g.boxes.n < 0 ||
       g.boxes.n > kMaxBoxes)
     return hipErrorInvalidValue;
-  AdjFwd<T> f[kAdjMaxFwd] = {};
-  for (int k = 0; k < nfwd; ++k) f[k] = fwd[k];
+  AdjArgs<T> a{};
+  a.g = g;
+  a.rev = rev;
+  for (int k = 0; k < nfwd; ++k) a.fwd[k] = fwd[k];
   const unsigned grid = adjoint_blocks(g.N) * static_cast<unsigned>((has_rev ? 1 : 0) + nfwd);
-  adjoint_kernel<T><<<grid, kAdjointThreads, 0, stream>>>(g, rev, f[0], f[1], f[2], has_rev ? 1 : 0);
+  if (has_rev)
+    adjoint_kernel<T, 1><<<grid, kAdjointThreads, 0, stream>>>(a);
+  else
+    adjoint_kernel<T, 0><<<grid, kAdjointThreads, 0, stream>>>(a);
   return hipGetLastError();
 }
 template hipError_t launch_adjoint<float>(const AdjGeom<float>&, const AdjRev<float>&, bool,

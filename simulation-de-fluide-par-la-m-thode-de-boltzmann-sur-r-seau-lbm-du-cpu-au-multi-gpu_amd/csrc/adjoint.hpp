@@ -1,8 +1,8 @@
 // Parameters and launcher of the discrete-adjoint kernels (adjoint.hip).
 //
-// One launch runs up to one reverse step and up to kMaxChains forward steps on disjoint
-// buffers (a checkpointing schedule packs a reverse step with the recomputations that do not
-// depend on it). Every step is one thread per cell with a single memory round trip: the cells
+// One launch runs up to one reverse step and up to three forward steps on disjoint buffers (a
+// checkpointing schedule packs a reverse step with the recomputations that do not depend on
+// it), each in its own blocks, one thread per cell, with a single memory round trip: the cells
 // that can need a bounce-back operand (solid cells, links to solids) lie inside a few "dirty
 // boxes" passed as kernel arguments, and only there the cell's own opposite populations are
 // loaded, together with everything else, before the link code is looked at.
@@ -17,7 +17,7 @@
 
 namespace lbmhip {
 
-constexpr int kMaxBoxes = 8;
+constexpr int kMaxBoxes = 4;
 constexpr int kAdjMaxFwd = 3;
 
 // Half-open boxes [x0, x1) x [y0, y1) covering every cell whose link code is not 0.
@@ -59,6 +59,14 @@ struct AdjRev {
   double* tacc;     // dJ/dtau accumulators, one FP64 per cell
   double* jacc;     // J accumulators per probe cell when R_t lies in the window, else null
   int direct;       // 1 if R_t lies in the averaging window (direct term of J)
+};
+
+// Everything one launch needs, passed by value.
+template <class T>
+struct AdjArgs {
+  AdjGeom<T> g;
+  AdjRev<T> rev;
+  AdjFwd<T> fwd[kAdjMaxFwd];
 };
 
 // Cells per block of the adjoint kernels (one thread per cell).
