@@ -258,54 +258,98 @@ __device__ __forceinline__ void reverse_finish(const AdjGeom<T>& G, const AdjRev
   p.tacc[c] = in.t0 + static_cast<double>(dtau);
 }
 
+// Tile of block b among the nb blocks of one role: each of the 8 XCDs walks a contiguous band
+// of tiles (neighbouring rows meet in its L2) when nb is a multiple of 8, so that block b of
+// every role runs on XCD b % 8.
+__device__ __forceinline__ unsigned tile_of(unsigned b, unsigned nb) {
+  return nb % kXcds == 0 ? xcd_tile(b, nb) : b;
+}
+
+// Where a thread of tile t sits, and what every role needs of its cell. A tile is
+// kAdjointThreads columns of one row, so the row index is uniform over the block.
+template <class T>
+struct CellAt {
+  int x, y;
+  unsigned c, pj, stride;
+  bool dirty, probe;
+  Nbr nb;
+  CellIn<T> cell;
+};
+
+// Returns false for a lane past the last column of the row.
+template <class T>
+__device__ __forceinline__ bool cell_at(const AdjGeom<T>& G, unsigned t, CellAt<T>& a) {
+  const unsigned tiles_x = (static_cast<unsigned>(G.nx) + kAdjointThreads - 1) / kAdjointThreads;
+  const unsigned ty = t / tiles_x;
+  a.y = static_cast<int>(ty);
+  a.x = static_cast<int>((t - ty * tiles_x) * kAdjointThreads + threadIdx.x);
+  if (a.x >= G.nx) return false;
+  a.c = static_cast<unsigned>(a.y) * static_cast<unsigned>(G.nx) + static_cast<unsigned>(a.x);
+  // Waves with a cell in a dirty box load the link codes and the cells' own populations (for
+  // every lane: valid addresses, and a clean cell's link code is 0); the others skip them.
+  a.dirty = __builtin_amdgcn_ballot_w64(in_boxes(G.boxes, a.x, a.y)) != 0;
+  a.probe = (a.x >= G.px0) & (a.x < G.px0 + G.pw) & (a.y >= G.py0) & (a.y < G.py0 + G.ph);
+  a.pj = static_cast<unsigned>((a.y - G.py0) * G.pw + (a.x - G.px0));
+  a.stride = static_cast<unsigned>(G.N * sizeof(T));
+  a.nb = neighbours<T>(a.x, a.y, G.nx, G.ny);
+  a.cell.link = 0u;
+  if (a.dirty) a.cell.link = G.links[a.c];
+  a.cell.fx = G.force[a.c];
+  a.cell.fy = G.force[G.N + a.c];
+  return true;
+}
+
+template <class T>
+__device__ __forceinline__ void forward_role(const AdjGeom<T>& G, CellAt<T>& a, const T* src,
+                                             T* dst, double* jacc) {
+  FwdIn<T> in;
+  forward_load(src, jacc, a.nb, a.stride, a.dirty, a.probe, a.pj, in);
+  pin(a.cell.link);
+  pin(a.cell.fx);
+  pin(a.cell.fy);
+  in.pin_all();
+  if (a.cell.link & kLinkSolid) return;
+  forward_finish(G, dst, jacc, a.nb, a.stride, a.dirty, a.probe, a.pj, a.cell, in);
+}
+
+template <class T>
+__device__ __forceinline__ void reverse_role(const AdjGeom<T>& G, CellAt<T>& a, const AdjRev<T>& r) {
+  const bool pr = a.probe && r.direct;
+  RevIn<T> in;
+  reverse_load(G, r, a.c, a.nb, a.stride, a.dirty, pr, a.pj, in);
+  pin(a.cell.link);
+  pin(a.cell.fx);
+  pin(a.cell.fy);
+  in.pin_all();
+  if (a.cell.link & kLinkSolid) return;
+  reverse_finish(G, r, a.c, a.nb, a.stride, a.dirty, pr, a.pj, a.cell, in);
+}
+
 // Blocks [0, nb) run the reverse step (kRev), the next nb blocks each forward step: the roles
 // run in different waves, each with a single memory round trip.
 template <class T, int kRev>
 __global__ void __launch_bounds__(kAdjointThreads) adjoint_kernel(const AdjArgs<T> a) {
   const AdjGeom<T>& G = a.g;
-  const unsigned nb = adjoint_blocks(G.N);
+  const unsigned nb = adjoint_blocks(G.nx, G.ny);
   const unsigned role = blockIdx.x / nb;
-  const unsigned c = (blockIdx.x - role * nb) * kAdjointThreads + threadIdx.x;
-  if (c >= G.N) return;
-  const unsigned nx = static_cast<unsigned>(G.nx);
-  const int y = static_cast<int>(c / nx), x = static_cast<int>(c - static_cast<unsigned>(y) * nx);
-  // Waves with a cell in a dirty box load the link codes and the cells' own populations (for
-  // every lane: valid addresses, and a clean cell's link code is 0); the others skip them.
-  const bool dirty = __builtin_amdgcn_ballot_w64(in_boxes(G.boxes, x, y)) != 0;
-  const bool probe = (x >= G.px0) & (x < G.px0 + G.pw) & (y >= G.py0) & (y < G.py0 + G.ph);
-  const unsigned pj = static_cast<unsigned>((y - G.py0) * G.pw + (x - G.px0));
-  const unsigned stride = static_cast<unsigned>(G.N * sizeof(T));
-  const Nbr nb3 = neighbours<T>(x, y, G.nx, G.ny);
-
-  CellIn<T> cell;
-  cell.link = 0u;
-  if (dirty) cell.link = G.links[c];
-  cell.fx = G.force[c];
-  cell.fy = G.force[G.N + c];
-  if (kRev && role == 0) {
-    const bool pr = probe && a.rev.direct;
-    RevIn<T> in;
-    reverse_load(G, a.rev, c, nb3, stride, dirty, pr, pj, in);
-    pin(cell.link);
-    pin(cell.fx);
-    pin(cell.fy);
-    in.pin_all();
-    if (cell.link & kLinkSolid) return;
-    reverse_finish(G, a.rev, c, nb3, stride, dirty, pr, pj, cell, in);
-    return;
-  }
+  CellAt<T> at;
+  if (!cell_at(G, tile_of(blockIdx.x - role * nb, nb), at)) return;
+  if (kRev && role == 0) return reverse_role(G, at, a.rev);
   const unsigned k = role - (kRev ? 1u : 0u);
   const T* src = k == 0 ? a.fwd[0].src : (k == 1 ? a.fwd[1].src : a.fwd[2].src);
   T* dst = k == 0 ? a.fwd[0].dst : (k == 1 ? a.fwd[1].dst : a.fwd[2].dst);
   double* jacc = k == 0 ? a.fwd[0].jacc : (k == 1 ? a.fwd[1].jacc : a.fwd[2].jacc);
-  FwdIn<T> in;
-  forward_load(src, jacc, nb3, stride, dirty, probe, pj, in);
-  pin(cell.link);
-  pin(cell.fx);
-  pin(cell.fy);
-  in.pin_all();
-  if (cell.link & kLinkSolid) return;
-  forward_finish(G, dst, jacc, nb3, stride, dirty, probe, pj, cell, in);
+  forward_role(G, at, src, dst, jacc);
+}
+
+// A launch holding one forward step only (the forward run, the first pass of the gradient):
+// the same per-cell code with a smaller argument block.
+template <class T>
+__global__ void __launch_bounds__(kAdjointThreads) adjoint_forward_kernel(const AdjGeom<T> G,
+                                                                          const AdjFwd<T> f) {
+  CellAt<T> at;
+  if (!cell_at(G, tile_of(blockIdx.x, adjoint_blocks(G.nx, G.ny)), at)) return;
+  forward_role(G, at, f.src, f.dst, f.jacc);
 }
 
 __global__ void __launch_bounds__(kReduceThreads) sum_f64_kernel(const double* __restrict__ a,
@@ -337,9 +381,11 @@ hipError_t launch_adjoint(const AdjGeom<T>& g, const AdjRev<T>& rev, bool has_re
   a.g = g;
   a.rev = rev;
   for (int k = 0; k < nfwd; ++k) a.fwd[k] = fwd[k];
-  const unsigned grid = adjoint_blocks(g.N) * static_cast<unsigned>((has_rev ? 1 : 0) + nfwd);
+  const unsigned grid = adjoint_blocks(g.nx, g.ny) * static_cast<unsigned>((has_rev ? 1 : 0) + nfwd);
   if (has_rev)
     adjoint_kernel<T, 1><<<grid, kAdjointThreads, 0, stream>>>(a);
+  else if (nfwd == 1)
+    adjoint_forward_kernel<T><<<grid, kAdjointThreads, 0, stream>>>(g, a.fwd[0]);
   else
     adjoint_kernel<T, 0><<<grid, kAdjointThreads, 0, stream>>>(a);
   return hipGetLastError();

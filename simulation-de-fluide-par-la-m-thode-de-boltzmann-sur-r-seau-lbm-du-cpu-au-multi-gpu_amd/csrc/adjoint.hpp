@@ -69,11 +69,12 @@ struct AdjArgs {
   AdjFwd<T> fwd[kAdjMaxFwd];
 };
 
-// Cells per block of the adjoint kernels (one thread per cell).
-constexpr int kAdjointThreads = 256;
+// Columns per block of the adjoint kernels (one thread per cell, one row per block).
+constexpr int kAdjointThreads = 64;
 
-__host__ __device__ inline unsigned adjoint_blocks(size_t N) {
-  return static_cast<unsigned>((N + kAdjointThreads - 1) / kAdjointThreads);
+// Blocks of one role (one step) of a launch.
+__host__ __device__ inline unsigned adjoint_blocks(int nx, int ny) {
+  return static_cast<unsigned>((nx + kAdjointThreads - 1) / kAdjointThreads) * static_cast<unsigned>(ny);
 }
 
 // A launch: the reverse step `rev` when has_rev, then nfwd forward steps fwd[0..nfwd).
