@@ -173,7 +173,7 @@ __device__ __forceinline__ bool gather_h(const GatherParamsH& p, int x, int y,
 
 __global__ void gather_fp16_kernel(const GatherParamsH p, float* __restrict__ out) {
   const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= p.N) return;
+  if (c >= static_cast<size_t>(p.nx) * p.ny) return;
   float v[kQ];
   if (!gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v))
 #pragma unroll
@@ -196,7 +196,7 @@ __global__ void macro_fp16_kernel(const GatherParamsH p, float fx, float fy,
                                   const float* __restrict__ force, float* __restrict__ rho,
                                   float* __restrict__ ux, float* __restrict__ uy) {
   const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= p.N) return;
+  if (c >= static_cast<size_t>(p.nx) * p.ny) return;
   float v[kQ];
   float r = 0.0f, u = 0.0f, w = 0.0f;
   if (gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v)) {
@@ -214,7 +214,8 @@ __global__ void macro_fp16_kernel(const GatherParamsH p, float fx, float fy,
 __global__ void __launch_bounds__(kReduceThreads)
     reduce_fp16_kernel(const GatherParamsH p, double* __restrict__ partials) {
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < p.N;
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < cells;
        c += static_cast<size_t>(gridDim.x) * kReduceThreads) {
     float v[kQ];
     if (!gather_h(p, static_cast<int>(c % p.nx), static_cast<int>(c / p.nx), v)) continue;
@@ -258,7 +259,8 @@ hipError_t launch_step_fp16(const StepParamsH& p, bool open_x, int block_x, hipS
 }
 
 hipError_t launch_gather_fp16(const GatherParamsH& p, float* out, hipStream_t stream) {
-  gather_fp16_kernel<<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, out);
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  gather_fp16_kernel<<<static_cast<unsigned>((cells + 255) / 256), 256, 0, stream>>>(p, out);
   return hipGetLastError();
 }
 
@@ -271,7 +273,8 @@ hipError_t launch_gather_one_fp16(const GatherParamsH& p, int x, int y, float* o
 hipError_t launch_macro_fp16(const GatherParamsH& p, float fx, float fy,
                              const float* force, float* rho, float* ux, float* uy,
                              hipStream_t stream) {
-  macro_fp16_kernel<<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, fx, fy, force,
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  macro_fp16_kernel<<<static_cast<unsigned>((cells + 255) / 256), 256, 0, stream>>>(p, fx, fy, force,
                                                                                   rho, ux, uy);
   return hipGetLastError();
 }

@@ -214,7 +214,7 @@ int lbm_get_state(lbm_solver* s, void* out) {
   LBM_TRY(check_handle(s));
   if (!out) return fail(LBM_EINVAL, "null output array");
   LBM_HIP_TRY(hipStreamSynchronize(s->stream));
-  LBM_HIP_TRY(hipMemcpy(out, s->g, s->state_bytes(), hipMemcpyDeviceToHost));
+  LBM_TRY(s->download_state(out, s->g));
   return LBM_OK;
 }
 
@@ -222,8 +222,8 @@ int lbm_set_state(lbm_solver* s, const void* in) {
   LBM_TRY(check_handle(s));
   if (!in) return fail(LBM_EINVAL, "null input array");
   LBM_HIP_TRY(hipStreamSynchronize(s->stream));
-  LBM_HIP_TRY(hipMemcpy(s->g, in, s->state_bytes(), hipMemcpyHostToDevice));
-  LBM_HIP_TRY(hipMemcpy(s->gn, in, s->state_bytes(), hipMemcpyHostToDevice));
+  LBM_TRY(s->upload_state(s->g, in));
+  LBM_TRY(s->upload_state(s->gn, in));
   return LBM_OK;
 }
 
@@ -243,11 +243,12 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
     std::snprintf(buf, len,
                   "\"gpu\": \"%s\", \"arch\": \"%s\", \"hip_runtime\": %d, \"hip_driver\": %d, "
                   "\"block\": \"%dx%d\", \"variant\": \"soa_pull_buffer\", \"storage\": \"%s\", "
-                  "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\"",
+                  "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\", "
+                  "\"plane_stride\": %zu",
                   prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
-                  s->xcd_bands ? "xcd_bands" : "round_robin");
+                  s->xcd_bands ? "xcd_bands" : "round_robin", s->S);
   return LBM_OK;
 }
 

@@ -26,7 +26,7 @@ StepParams<T> make_step_params(const SolverState& s) {
   p.links = s.links;
   p.segflags = s.segflags;
   p.force = static_cast<const T*>(s.force);
-  p.N = s.cells();
+  p.N = s.S;
   p.nx = s.nx;
   p.ny = s.ny;
   p.row0 = 0;
@@ -57,7 +57,7 @@ GatherParams<T> make_gather_params(const SolverState& s) {
   p.g = static_cast<const T*>(s.g);
   p.links = s.links;
   p.force = static_cast<const T*>(s.force);
-  p.N = s.cells();
+  p.N = s.S;
   p.nx = s.nx;
   p.ny = s.ny;
   p.fx = static_cast<T>(s.fx);
@@ -72,7 +72,7 @@ GatherParamsH make_gather_params_h(const SolverState& s) {
   GatherParamsH p{};
   p.g = static_cast<const uint16_t*>(s.g);
   p.links = s.links;
-  p.N = s.cells();
+  p.N = s.S;
   p.nx = s.nx;
   p.ny = s.ny;
   float eq[kQ] = {};
@@ -89,7 +89,7 @@ StepParamsH make_step_params_h(const SolverState& s) {
   p.links = s.links;
   p.segflags = s.segflags;
   p.force = static_cast<const float*>(s.force);
-  p.N = s.cells();
+  p.N = s.S;
   p.nx = s.nx;
   p.ny = s.ny;
   p.row0 = 0;
@@ -124,7 +124,29 @@ uint16_t to_half(float f) {
 
 }  // namespace
 
-size_t SolverState::state_bytes() const { return cells() * kQ * storage_size(prec); }
+size_t plane_stride(size_t N, size_t elem_bytes) {
+  if (N * elem_bytes < (size_t(1) << 20)) return N;
+  const size_t q = 65536 / elem_bytes;
+  return (N + q - 1) / q * q + 4352 / elem_bytes;
+}
+
+size_t SolverState::state_bytes() const { return S * kQ * storage_size(prec); }
+
+int SolverState::upload_state(void* dev, const void* host) {
+  const size_t es = storage_size(prec), row = cells() * es;
+  for (int i = 0; i < kQ; ++i)
+    LBM_HIP_TRY(hipMemcpy(static_cast<char*>(dev) + i * S * es,
+                          static_cast<const char*>(host) + i * row, row, hipMemcpyHostToDevice));
+  return LBM_OK;
+}
+
+int SolverState::download_state(void* host, const void* dev) {
+  const size_t es = storage_size(prec), row = cells() * es;
+  for (int i = 0; i < kQ; ++i)
+    LBM_HIP_TRY(hipMemcpy(static_cast<char*>(host) + i * row,
+                          static_cast<const char*>(dev) + i * S * es, row, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
 
 int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, const uint8_t* solid_,
                         int prec_) {
@@ -134,7 +156,8 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
   fx = fx_;
   fy = fy_;
   LBM_TRY(set_tau(tau_));
-  if (storage_size(prec) * cells() >= (size_t(1) << 32))
+  S = plane_stride(cells(), storage_size(prec));
+  if (storage_size(prec) * S >= (size_t(1) << 32))
     return fail(LBM_EINVAL, "one population plane must stay below 4 GiB");
   solid.assign(solid_, solid_ + cells());
   // Launch defaults from shape scans of the 8192^2 channel on MI355X: one row per thread and
@@ -230,10 +253,16 @@ int SolverState::set_force_field(const void* fxy) {
     force = nullptr;
     return LBM_OK;
   }
-  const size_t bytes = 2 * cells() * real_size(prec);
-  if (!force && hipMalloc(&force, bytes) != hipSuccess)
-    return fail(LBM_ENOMEM, "device allocation of the force field failed");
-  LBM_HIP_TRY(hipMemcpy(force, fxy, bytes, hipMemcpyHostToDevice));
+  // fx plane then fy plane, S values apart like the population planes
+  const size_t rs = real_size(prec), row = cells() * rs;
+  if (!force) {
+    if (hipMalloc(&force, 2 * S * rs) != hipSuccess)
+      return fail(LBM_ENOMEM, "device allocation of the force field failed");
+    LBM_HIP_TRY(hipMemset(force, 0, 2 * S * rs));
+  }
+  LBM_HIP_TRY(hipMemcpy(force, fxy, row, hipMemcpyHostToDevice));
+  LBM_HIP_TRY(hipMemcpy(static_cast<char*>(force) + S * rs, static_cast<const char*>(fxy) + row, row,
+                        hipMemcpyHostToDevice));
   return LBM_OK;
 }
 
@@ -241,9 +270,8 @@ int SolverState::init_equilibrium(double rho0, double ux0, double uy0) {
   const size_t n = cells() * kQ;
   auto upload = [&](const void* host) -> int {
     LBM_HIP_TRY(hipStreamSynchronize(stream));
-    LBM_HIP_TRY(hipMemcpy(g, host, state_bytes(), hipMemcpyHostToDevice));
-    LBM_HIP_TRY(hipMemcpy(gn, host, state_bytes(), hipMemcpyHostToDevice));
-    return LBM_OK;
+    LBM_TRY(upload_state(g, host));
+    return upload_state(gn, host);
   };
   if (prec == LBM_DOUBLE) {
     double eq[kQ];
@@ -269,9 +297,8 @@ int SolverState::set_populations(const void* in) {
   const size_t n = cells() * kQ;
   auto upload = [&](const void* host) -> int {
     LBM_HIP_TRY(hipStreamSynchronize(stream));
-    LBM_HIP_TRY(hipMemcpy(g, host, state_bytes(), hipMemcpyHostToDevice));
-    LBM_HIP_TRY(hipMemcpy(gn, host, state_bytes(), hipMemcpyHostToDevice));
-    return LBM_OK;
+    LBM_TRY(upload_state(g, host));
+    return upload_state(gn, host);
   };
   if (prec == LBM_DOUBLE) {
     std::vector<double> R(n);
@@ -399,13 +426,13 @@ int SolverState::momentum_exchange(int x0, int y0, int w, int h, double* fx, dou
   double* out = red + 4 * kReduceBlocks;
   hipError_t e;
   if (prec == LBM_FLOAT)
-    e = launch_exchange(static_cast<const float*>(g), links, cells(), nx, x0, y0, w, h, 1.0f, out,
+    e = launch_exchange(static_cast<const float*>(g), links, S, nx, x0, y0, w, h, 1.0f, out,
                         stream);
   else if (prec == LBM_DOUBLE)
-    e = launch_exchange(static_cast<const double*>(g), links, cells(), nx, x0, y0, w, h, 1.0f, out,
+    e = launch_exchange(static_cast<const double*>(g), links, S, nx, x0, y0, w, h, 1.0f, out,
                         stream);
   else
-    e = launch_exchange(static_cast<const uint16_t*>(g), links, cells(), nx, x0, y0, w, h,
+    e = launch_exchange(static_cast<const uint16_t*>(g), links, S, nx, x0, y0, w, h,
                         static_cast<float>(1.0 / fp16_scale), out, stream);
   if (e != hipSuccess) return fail(LBM_EHIP, std::string("exchange launch: ") + hipGetErrorString(e));
   LBM_HIP_TRY(hipStreamSynchronize(stream));

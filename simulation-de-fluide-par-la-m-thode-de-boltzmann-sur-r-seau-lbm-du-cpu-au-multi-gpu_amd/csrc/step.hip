@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
 template <class T>
 __global__ void gather_kernel(const GatherParams<T> p, T* __restrict__ out) {
   const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= p.N) return;
+  if (c >= static_cast<size_t>(p.nx) * p.ny) return;
   const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
   T v[kQ];
   if (!gather_cell<T>(p.g, p.links, p.N, p.nx, p.ny, p.eq_in, x, y, v))
@@ -68,7 +68,7 @@ template <class T>
 __global__ void macro_kernel(const GatherParams<T> p, T* __restrict__ rho, T* __restrict__ ux,
                              T* __restrict__ uy) {
   const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= p.N) return;
+  if (c >= static_cast<size_t>(p.nx) * p.ny) return;
   const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
   T v[kQ];
   T r = T(0), u = T(0), w = T(0);
@@ -86,7 +86,8 @@ template <class T>
 __global__ void __launch_bounds__(kReduceThreads) reduce_kernel(const GatherParams<T> p,
                                                                 double* __restrict__ partials) {
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < p.N;
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  for (size_t c = static_cast<size_t>(blockIdx.x) * kReduceThreads + threadIdx.x; c < cells;
        c += static_cast<size_t>(gridDim.x) * kReduceThreads) {
     const int x = static_cast<int>(c % p.nx), y = static_cast<int>(c / p.nx);
     T v[kQ];
@@ -189,7 +190,8 @@ template hipError_t launch_step<double>(const StepParams<double>&, bool, int, hi
 
 template <class T>
 hipError_t launch_gather(const GatherParams<T>& p, T* out, hipStream_t stream) {
-  gather_kernel<T><<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, out);
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  gather_kernel<T><<<static_cast<unsigned>((cells + 255) / 256), 256, 0, stream>>>(p, out);
   return hipGetLastError();
 }
 template hipError_t launch_gather<float>(const GatherParams<float>&, float*, hipStream_t);
@@ -207,7 +209,8 @@ template hipError_t launch_gather_one<double>(const GatherParams<double>&, int, 
 
 template <class T>
 hipError_t launch_macro(const GatherParams<T>& p, T* rho, T* ux, T* uy, hipStream_t stream) {
-  macro_kernel<T><<<static_cast<unsigned>((p.N + 255) / 256), 256, 0, stream>>>(p, rho, ux, uy);
+  const size_t cells = static_cast<size_t>(p.nx) * p.ny;
+  macro_kernel<T><<<static_cast<unsigned>((cells + 255) / 256), 256, 0, stream>>>(p, rho, ux, uy);
   return hipGetLastError();
 }
 template hipError_t launch_macro<float>(const GatherParams<float>&, float*, float*, float*,
