@@ -81,12 +81,14 @@ int lbm_get_block(const lbm_solver* s, int* block_x, int* rows_per_thread);
 /* No reference counterpart: code-generation options of the step (same bits either way;
  * lbm_autotune tries them): streaming_stores = non-temporal stores (periodic, uniform-force
  * kernel); single_resource = one buffer resource for the nine planes when they span < 4 GiB
- * (fewer scalar registers); xcd_bands = workgroups remapped so that each of the 8 XCDs walks
- * a contiguous band of tiles (0: the hardware's round-robin order). Defaults: fp32 1, 1, 0;
+ * (fewer scalar registers); tile_order = the order in which workgroups take tiles: 0 the
+ * hardware's round-robin over the 8 XCDs, 1 a contiguous band of tiles per XCD, 2 whole tile
+ * rows per XCD (row r on XCD r % 8: the cache lines that neighbouring tiles of a row share stay
+ * in one L2; needs a multiple of 8 tile rows, else round-robin). Defaults: fp32 1, 1, 0;
  * fp64 1, 1, 1; FP16 0, 1, 0. */
-int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int xcd_bands);
+int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int tile_order);
 int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
-                           int* xcd_bands);
+                           int* tile_order);
 
 /* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */
 int lbm_step(lbm_solver* s);

@@ -365,14 +365,16 @@ class Solver:
         check(lib().lbm_set_block(self._h, int(block_x), int(rows_per_thread)))
 
     def set_kernel_options(self, streaming_stores: bool, single_resource: bool,
-                           xcd_bands: bool = False):
+                           tile_order: int = 0):
+        """tile_order: 0 round-robin over the XCDs, 1 a band of tiles per XCD, 2 whole tile
+        rows per XCD."""
         check(lib().lbm_set_kernel_options(self._h, int(bool(streaming_stores)),
-                                           int(bool(single_resource)), int(bool(xcd_bands))))
+                                           int(bool(single_resource)), int(tile_order)))
 
-    def kernel_options(self) -> tuple[bool, bool, bool]:
+    def kernel_options(self) -> tuple[bool, bool, int]:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         check(lib().lbm_get_kernel_options(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return bool(a.value), bool(b.value), bool(c.value)
+        return bool(a.value), bool(b.value), c.value
 
     def block(self) -> tuple[int, int]:
         bx, r = C.c_int(), C.c_int()

@@ -21,6 +21,22 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
 }
 
+// Tile of workgroup b when every XCD takes whole rows of tiles: XCD k = b % 8 walks tile rows
+// k, k + 8, k + 16, ... So all tiles of a row, whose edge cache lines the +-1 column shift of the
+// pull shares (the periodic ends included), meet in one L2, while the eight XCDs stay on eight
+// adjacent rows (the DRAM pages of the round-robin order). Needs a multiple of 8 tile rows.
+__device__ __forceinline__ unsigned xcd_row_tile(unsigned b, unsigned tiles_x) {
+  const unsigned xcd = b % kXcds, k = b / kXcds;
+  const unsigned group = k / tiles_x, tx = k - group * tiles_x;
+  return (group * kXcds + xcd) * tiles_x + tx;
+}
+
+// Tile of workgroup b for a tile order: 0 hardware round-robin, 1 XCD bands, 2 XCD rows.
+__device__ __forceinline__ unsigned tile_in_order(int order, unsigned b, unsigned nb,
+                                                  unsigned tiles_x) {
+  return order == 1 ? xcd_tile(b, nb) : (order == 2 ? xcd_row_tile(b, tiles_x) : b);
+}
+
 // ---- buffer addressing ----------------------------------------------------------------------
 // Each population plane gets its own buffer resource (4 SGPRs). A load is
 //   buffer_load v, v_col, s[plane], s_row offen

@@ -110,7 +110,7 @@ __device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ
 template <bool kOpen, bool kField, bool kOne>
 __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
   using PS = std::conditional_t<kOne, PlanesOne, Planes>;
-  const unsigned tile = p.xcd_bands ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, static_cast<unsigned>(p.tiles_x));
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const int nx = p.nx;

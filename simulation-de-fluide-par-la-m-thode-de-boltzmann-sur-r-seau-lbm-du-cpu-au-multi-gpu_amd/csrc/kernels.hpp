@@ -40,7 +40,8 @@ struct StepParams {
   T eq_in[9];          // inlet equilibrium (open x)
   int store_nt;        // 1: non-temporal stores (periodic, uniform-force kernel only)
   int one_rsrc;        // 1: one buffer resource for the nine planes when they fit in 4 GiB
-  int xcd_bands;       // 1: workgroup ids remapped to contiguous tile bands per XCD
+  int xcd_bands;       // tile order: 0 round-robin, 1 contiguous bands per XCD, 2 tile rows
+                       // per XCD (row r on XCD r % 8; needs a multiple of 8 tile rows)
 };
 
 // Collision constants of the FP16-storage step (scaled units g' = S g), computed once per launch
@@ -92,7 +93,7 @@ struct StepParamsH {
   float scale, inv_scale;
   float eq_in[9];  // inlet equilibrium, already scaled
   uint32_t seed;   // stochastic-rounding stream of this step
-  int xcd_bands;   // 1: workgroup ids remapped to contiguous tile bands per XCD
+  int xcd_bands;   // tile order, as StepParams::xcd_bands
   HalfCollide col; // uniform-force collision constants
 };
 

@@ -117,20 +117,21 @@ int lbm_set_block(lbm_solver* s, int block_x, int rows_per_thread) {
   return LBM_OK;
 }
 
-int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int xcd_bands) {
+int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int tile_order) {
   LBM_TRY(check_handle(s));
+  if (tile_order < 0 || tile_order > 2) return fail(LBM_EINVAL, "tile_order must be 0, 1 or 2");
   s->store_nt = streaming_stores ? 1 : 0;
   s->one_rsrc = single_resource ? 1 : 0;
-  s->xcd_bands = xcd_bands ? 1 : 0;
+  s->xcd_bands = tile_order;
   return LBM_OK;
 }
 
 int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
-                           int* xcd_bands) {
+                           int* tile_order) {
   LBM_TRY(check_handle(s));
   if (streaming_stores) *streaming_stores = s->store_nt;
   if (single_resource) *single_resource = s->one_rsrc;
-  if (xcd_bands) *xcd_bands = s->xcd_bands;
+  if (tile_order) *tile_order = s->xcd_bands;
   return LBM_OK;
 }
 
@@ -248,7 +249,8 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
                   prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
-                  s->xcd_bands ? "xcd_bands" : "round_robin", s->S);
+                  s->xcd_bands == 1 ? "xcd_bands" : (s->xcd_bands == 2 ? "xcd_rows" : "round_robin"),
+                  s->S);
   return LBM_OK;
 }
 

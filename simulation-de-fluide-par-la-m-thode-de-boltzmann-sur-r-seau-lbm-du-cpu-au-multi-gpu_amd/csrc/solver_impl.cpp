@@ -47,7 +47,7 @@ StepParams<T> make_step_params(const SolverState& s) {
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i];
   p.store_nt = s.store_nt;
   p.one_rsrc = s.one_rsrc;
-  p.xcd_bands = s.xcd_bands;
+  p.xcd_bands = s.tile_order_for(s.rows);
   return p;
 }
 
@@ -109,7 +109,7 @@ StepParamsH make_step_params_h(const SolverState& s) {
   if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i] * p.scale;
   p.seed = s.step_count * 0x9E3779B9u + 0x7F4A7C15u;
-  p.xcd_bands = s.xcd_bands;
+  p.xcd_bands = s.tile_order_for(s.rows);
   p.col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
   return p;
 }
@@ -456,7 +456,7 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
   const uint32_t keep_count = step_count;  // FP16 rounding seeds stay as if no trial ran
   double best = -1.0;
   int bbx = keep_bx, brows = keep_rows, bnt = store_nt, bone = one_rsrc, bband = xcd_bands;
-  for (int band : {0, 1})
+  for (int band : {0, 1, 2})
   for (int one : {1, 0})
   for (int nt : {0, 1})
   for (int bx : {128, 256, 512, 1024})

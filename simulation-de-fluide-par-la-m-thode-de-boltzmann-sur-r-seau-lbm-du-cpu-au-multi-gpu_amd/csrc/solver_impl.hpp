@@ -17,7 +17,12 @@ struct SolverState {
   int block_x = 256, rows = 8;  // launch shape (tile = block_x columns x rows)
   int store_nt = 0;             // non-temporal stores in the periodic uniform-force step
   int one_rsrc = 1;             // one buffer resource for all planes (when < 4 GiB)
-  int xcd_bands = 0;            // 1: each XCD walks a contiguous band of tiles
+  int xcd_bands = 0;            // tile order: 0 round-robin, 1 XCD bands, 2 XCD rows
+  // The tile order a launch uses: XCD rows need a multiple of 8 tile rows, else round-robin.
+  int tile_order_for(int rows_per_tile) const {
+    const int tiles_y = (ny + rows_per_tile - 1) / rows_per_tile;
+    return xcd_bands == 2 && tiles_y % 8 != 0 ? 0 : xcd_bands;
+  }
   double fp16_scale = 4096.0;   // stored half = g * scale (LBM_FP16)
   uint32_t step_count = 0;      // steps taken (seeds the FP16 stochastic rounding)
 

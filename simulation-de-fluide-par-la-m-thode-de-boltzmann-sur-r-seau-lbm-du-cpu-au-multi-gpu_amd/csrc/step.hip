@@ -27,7 +27,7 @@ namespace {
 
 template <class T, bool kOpen, bool kField, int kStoreAux, bool kOne = false>
 __global__ void __launch_bounds__(1024) step_kernel(const StepParams<T> p) {
-  const unsigned tile = p.xcd_bands ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, static_cast<unsigned>(p.tiles_x));
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const int x = tx * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
