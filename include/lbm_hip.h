@@ -85,7 +85,7 @@ int lbm_get_block(const lbm_solver* s, int* block_x, int* rows_per_thread);
  * hardware's round-robin over the 8 XCDs, 1 a contiguous band of tiles per XCD, 2 whole tile
  * rows per XCD (row r on XCD r % 8: the cache lines that neighbouring tiles of a row share stay
  * in one L2; needs a multiple of 8 tile rows, else round-robin). Defaults: fp32 1, 1, 0;
- * fp64 1, 1, 1; FP16 0, 1, 0. */
+ * fp64 1, 1, 1; FP16 0, 1, 2. */
 int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int tile_order);
 int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
                            int* tile_order);

@@ -153,6 +153,11 @@ template <class S>
 hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,
                            int w, int h, float inv_scale, double* out, hipStream_t stream);
 
+// Strip ring closed on itself: ghost rows 0 and nyl + 1 of the state g (plane stride N, rows
+// of nx values) refreshed from the owned boundary rows, one launch.
+template <class T>
+hipError_t launch_halo_ring(T* g, size_t N, int nx, int nyl, hipStream_t stream);
+
 // Grid-stride copy (HBM ceiling), n floats.
 // variant 0: scalar (the reference's kernel), 1: 16-byte accesses, 2: 16-byte non-temporal.
 hipError_t launch_copy(const float* a, float* b, size_t n, int grid, int block, int variant,

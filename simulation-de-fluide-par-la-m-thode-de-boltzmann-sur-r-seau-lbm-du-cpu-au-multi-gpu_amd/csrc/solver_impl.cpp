@@ -162,14 +162,15 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
   solid.assign(solid_, solid_ + cells());
   // Launch defaults from shape scans of the 8192^2 channel on MI355X: one row per thread and
   // non-temporal stores for fp32/fp64 (512 x 1: more waves in flight, no per-wave row loop);
-  // FP16 prefers 256 x 4 with ordinary stores. Per-XCD tile bands: about +12 % for fp64, equal
-  // for fp32, up to 8 % slower for FP16 than the hardware's round-robin order.
+  // FP16 prefers 256 x 4 with ordinary stores. Tile order: XCD bands about +12 % for fp64;
+  // fp32 within noise of round-robin for every order; FP16 fastest with XCD rows.
   // LBM_BLOCK overrides the shape.
   if (prec == LBM_FP16) {
     block_x = 256;
     rows = 4;
     store_nt = 0;
-    xcd_bands = 0;
+    xcd_bands = 2;  // XCD rows: 5.21 vs 4.90-5.17 TB/s round-robin (the 2-byte tiles share the
+                    // most edge cache lines per byte)
   } else {
     block_x = 512;
     rows = 1;

@@ -253,10 +253,9 @@ int lbm_strip_set_block(lbm_strip* s, int block_x, int rows_per_thread) {
 
 int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds) {
   // The strip at its real shape with the ring closed on itself: after each update the ghost
-  // rows of the new state are refreshed from the strip's own boundary rows by device-to-device
-  // copies (what an RCCL/xGMI exchange would deliver), all on stream 0.
+  // rows of the new state are refreshed from the strip's own boundary rows by one device kernel
+  // (the contents an RCCL/xGMI exchange would deliver), all on stream 0.
   if (!s || steps < 0) return fail(LBM_EINVAL, "null strip handle or negative steps");
-  const size_t row = static_cast<size_t>(s->nx) * s->rs();
   hipStream_t st = s->streams[0];
   for (auto x : s->streams) LBM_HIP_TRY(hipStreamSynchronize(x));
   LBM_HIP_TRY(hipEventRecord(s->ev0, st));
@@ -265,14 +264,10 @@ int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds) {
                              ? launch_step(s->params<double>(1, s->nyl), s->open_x, s->block_x, st)
                              : launch_step(s->params<float>(1, s->nyl), s->open_x, s->block_x, st);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("strip launch: ") + hipGetErrorString(e));
-    for (int j = 0; j < 3; ++j) {
-      LBM_HIP_TRY(hipMemcpyAsync(s->plane_row(s->gn, kHaloUp[j], 0),
-                                 s->plane_row(s->gn, kHaloUp[j], s->nyl), row,
-                                 hipMemcpyDeviceToDevice, st));
-      LBM_HIP_TRY(hipMemcpyAsync(s->plane_row(s->gn, kHaloDown[j], s->nyl + 1),
-                                 s->plane_row(s->gn, kHaloDown[j], 1), row,
-                                 hipMemcpyDeviceToDevice, st));
-    }
+    const hipError_t h = s->prec == LBM_DOUBLE
+                             ? launch_halo_ring(static_cast<double*>(s->gn), s->N(), s->nx, s->nyl, st)
+                             : launch_halo_ring(static_cast<float*>(s->gn), s->N(), s->nx, s->nyl, st);
+    if (h != hipSuccess) return fail(LBM_EHIP, std::string("halo launch: ") + hipGetErrorString(h));
     std::swap(s->g, s->gn);
   }
   LBM_HIP_TRY(hipEventRecord(s->ev1, st));
