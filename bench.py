@@ -129,16 +129,18 @@ def form_batched(lbm, args):
     doms = lbm.random_domains(seed=0, n=256)
     b = lbm.Batch(doms, "float")
     tx, tr, wg, tuned = b.autotune(steps=3)
+    order, nt = b.order()
     b.set_graph_steps(100)
     b.run(100)
     t = b.timed_run(1000)
     cells = b.total_cells()
     mlups = cells * 1000 / t / 1e6
     b.close()
-    log(f"batched: {mlups:.0f} MLUPS over {cells} cells (tile {tx}x{tr}, {wg} WGs)")
+    log(f"batched: {mlups:.0f} MLUPS over {cells} cells (order {order}, tile {tx}x{tr}, {wg} WGs)")
     return {"domains": 256, "seed": 0, "cells": cells, "mlups": round(mlups, 1),
-            "timed_steps": 1000, "graph_steps": 100, "tile_x": tx, "tile_rows": tr,
-            "workgroups": wg, "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / 1e9, 1)}
+            "timed_steps": 1000, "graph_steps": 100, "order": order, "streaming_stores": nt,
+            "tile_x": tx, "tile_rows": tr, "workgroups": wg,
+            "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / 1e9, 1)}
 
 
 def form_strip(lbm, args):

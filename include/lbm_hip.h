@@ -182,12 +182,21 @@ int lbm_batch_create(int n_domains, const lbm_domain_desc* domains, int precisio
 int lbm_batch_destroy(lbm_batch* b);
 /* Steps per captured HIP graph (default 100); the graph is rebuilt on change. */
 int lbm_batch_set_graph_steps(lbm_batch* b, int steps_per_graph);
-/* Tile width (multiple of 64), rows per tile and workgroups of the persistent launch. */
+/* Tile width (multiple of 64; in order 2 the threads per workgroup), rows per tile and
+ * workgroups of the persistent launch. */
 int lbm_batch_set_shape(lbm_batch* b, int tile_x, int tile_rows, int workgroups);
+/* How the workgroups walk the work: 0 tiles w, w + G, ... (G workgroups); 1 a contiguous chunk
+ * of tiles per workgroup, workgroups banded per XCD; 2 a wave work-list (every wavefront takes
+ * 64-column segments of one domain over tile_rows rows: narrow domains leave few lanes idle).
+ * streaming_stores: non-temporal stores. Every order computes the same bits. */
+int lbm_batch_set_order(lbm_batch* b, int order, int streaming_stores);
+int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores);
 int lbm_batch_run(lbm_batch* b, int steps);
 int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds);
 int lbm_batch_populations(lbm_batch* b, int domain, void* out);
 int lbm_batch_total_cells(const lbm_batch* b, long long* cells);
+/* Times every (order, stores, tile shape, workgroups) candidate for `steps` steps on a copy of
+ * the state and keeps the fastest (lbm_batch_get_order reports its order). */
 int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
                        double* best_mlups);
 

@@ -114,6 +114,8 @@ def _declare(L):
         "lbm_batch_populations": ([vp, i, vp], i),
         "lbm_batch_total_cells": ([vp, llp], i),
         "lbm_batch_autotune": ([vp, i, ip, ip, ip, dp], i),
+        "lbm_batch_set_order": ([vp, i, i], i),
+        "lbm_batch_get_order": ([vp, ip, ip], i),
         "lbm_adjoint_create": ([i, i, d, _u8p, i, i, i, i, i, i, i, i, C.POINTER(vp)], i),
         "lbm_adjoint_destroy": ([vp], i),
         "lbm_adjoint_set_force_field": ([vp, vp], i),
@@ -534,6 +536,15 @@ class Batch:
 
     def set_shape(self, tile_x, tile_rows, workgroups):
         check(lib().lbm_batch_set_shape(self._h, int(tile_x), int(tile_rows), int(workgroups)))
+
+    def set_order(self, order: int, streaming_stores: bool = True):
+        """0 tiles w, w + G, ...; 1 contiguous chunks banded per XCD; 2 wave work-list."""
+        check(lib().lbm_batch_set_order(self._h, int(order), int(bool(streaming_stores))))
+
+    def order(self) -> tuple[int, bool]:
+        o, nt = C.c_int(), C.c_int()
+        check(lib().lbm_batch_get_order(self._h, C.byref(o), C.byref(nt)))
+        return o.value, bool(nt.value)
 
     def run(self, steps):
         check(lib().lbm_batch_run(self._h, int(steps)))

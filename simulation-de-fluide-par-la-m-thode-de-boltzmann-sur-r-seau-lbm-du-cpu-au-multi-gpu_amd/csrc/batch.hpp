@@ -22,6 +22,15 @@ struct BatchTile {
   int domain, tx, ty, pad;
 };
 
+// Work item of the wave work-list (order 2): one 64-column segment of one domain over rows
+// [y0, y1), taken by one wavefront.
+struct BatchSeg {
+  int domain, seg, y0, y1;
+};
+
+// Domains whose descriptors the wave work-list kernel keeps in LDS (more: read from memory).
+constexpr int kBatchLdsDomains = 256;
+
 template <class T>
 struct BatchParams {
   const T* g;
@@ -30,11 +39,14 @@ struct BatchParams {
   const uint8_t* segflags;
   const BatchDomain<T>* domains;
   const BatchTile* tiles;
-  int ntiles;
+  const BatchSeg* segs;  // order 2
+  int ndomains;
+  int ntiles;            // tiles (orders 0, 1) or segments (order 2)
   int tile_rows;
   int chunk;     // order 1: tiles per workgroup
   int order;     // 0: workgroup w takes tiles w, w + G, w + 2G, ... (G = grid size);
-                 // 1: a contiguous chunk of tiles, workgroups in per-XCD bands
+                 // 1: a contiguous chunk of tiles, workgroups in per-XCD bands;
+                 // 2: wave work-list, wave v of the grid takes segments v, v + V, ...
   int store_nt;  // 1: non-temporal stores
   int one_rsrc;  // 1: every domain's nine planes span < 4 GiB (one buffer resource each)
 };

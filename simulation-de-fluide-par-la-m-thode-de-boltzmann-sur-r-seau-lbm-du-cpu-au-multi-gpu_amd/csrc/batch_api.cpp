@@ -42,7 +42,9 @@ struct lbm_batch {
   void* ddesc = nullptr;
   BatchTile* dtiles = nullptr;
   size_t tiles_cap = 0;
-  int ntiles = 0;
+  BatchSeg* dsegs = nullptr;  // wave work-list (order 2)
+  size_t segs_cap = 0;
+  int ntiles = 0;             // entries of the list the current order walks
   int tile_x = 256, tile_rows = 4, workgroups = 0;
   int order = 0, store_nt = 1;  // tile order (see batch.hip) and non-temporal stores
   bool one_rsrc = true;         // every domain's nine planes span < 4 GiB
@@ -68,7 +70,7 @@ struct lbm_batch {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (void* p : {g, gn, static_cast<void*>(links), static_cast<void*>(segflags), ddesc,
-                    static_cast<void*>(dtiles), staging})
+                    static_cast<void*>(dtiles), static_cast<void*>(dsegs), staging})
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -99,7 +101,29 @@ struct lbm_batch {
     return LBM_OK;
   }
 
+  // Order 2: per-wavefront segments of 64 columns over tile_rows rows, domain by domain, row
+  // block by row block (neighbouring segments go to neighbouring waves of one workgroup).
+  int build_segs() {
+    std::vector<BatchSeg> t;
+    for (size_t k = 0; k < doms.size(); ++k)
+      for (int y = 0; y < doms[k].ny; y += tile_rows)
+        for (int sg = 0; sg < doms[k].nseg; ++sg)
+          t.push_back(BatchSeg{static_cast<int>(k), sg, y, std::min(y + tile_rows, doms[k].ny)});
+    ntiles = static_cast<int>(t.size());
+    if (t.size() > segs_cap) {
+      if (dsegs) LBM_HIP_TRY(hipFree(dsegs));
+      dsegs = nullptr;
+      if (hipMalloc(&dsegs, t.size() * sizeof(BatchSeg)) != hipSuccess)
+        return fail(LBM_ENOMEM, "device allocation of the segment list failed");
+      segs_cap = t.size();
+    }
+    LBM_HIP_TRY(hipMemcpy(dsegs, t.data(), t.size() * sizeof(BatchSeg), hipMemcpyHostToDevice));
+    drop_graphs();
+    return LBM_OK;
+  }
+
   int build_tiles() {
+    if (order == 2) return build_segs();
     std::vector<BatchTile> t;
     for (size_t k = 0; k < doms.size(); ++k) {
       const int tx = (doms[k].nx + tile_x - 1) / tile_x, ty = (doms[k].ny + tile_rows - 1) / tile_rows;
@@ -128,9 +152,11 @@ struct lbm_batch {
     p.segflags = segflags;
     p.domains = static_cast<const BatchDomain<T>*>(ddesc);
     p.tiles = dtiles;
+    p.segs = dsegs;
+    p.ndomains = static_cast<int>(doms.size());
     p.ntiles = ntiles;
     p.tile_rows = tile_rows;
-    const int wg = std::max(1, std::min(workgroups, ntiles));
+    const int wg = std::max(1, std::min(workgroups, order == 2 ? (ntiles + tile_x / 64 - 1) / (tile_x / 64) : ntiles));
     p.chunk = (ntiles + wg - 1) / wg;
     p.order = order;
     p.store_nt = store_nt;
@@ -139,7 +165,8 @@ struct lbm_batch {
   }
 
   int launch(bool flip) {
-    const int wg = std::max(1, std::min(workgroups, ntiles));
+    // order 2 needs no more workgroups than it has waves' worth of segments
+    const int wg = std::max(1, std::min(workgroups, order == 2 ? (ntiles + tile_x / 64 - 1) / (tile_x / 64) : ntiles));
     const hipError_t e = prec == LBM_DOUBLE
                              ? launch_batch(params<double>(flip), tile_x, wg, stream)
                              : launch_batch(params<float>(flip), tile_x, wg, stream);
@@ -303,6 +330,22 @@ int lbm_batch_set_shape(lbm_batch* b, int tile_x, int tile_rows, int workgroups)
   return b->build_tiles();
 }
 
+int lbm_batch_set_order(lbm_batch* b, int order, int streaming_stores) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  if (order < 0 || order > 2) return fail(LBM_EINVAL, "order must be 0, 1 or 2");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->order = order;
+  b->store_nt = streaming_stores ? 1 : 0;
+  return b->build_tiles();
+}
+
+int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  if (order) *order = b->order;
+  if (streaming_stores) *streaming_stores = b->store_nt;
+  return LBM_OK;
+}
+
 int lbm_batch_run(lbm_batch* b, int steps) {
   if (!b || steps < 0) return fail(LBM_EINVAL, "null batch handle or negative steps");
   LBM_TRY(b->run_async(steps));
@@ -391,7 +434,7 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
   double best = -1.0;
   int btx = b->tile_x, brows = b->tile_rows, bwg = b->workgroups, border = b->order,
       bnt = b->store_nt;
-  for (int ord : {0, 1})
+  for (int ord : {0, 1, 2})
   for (int nt : {1, 0})
   for (int tx : {128, 256, 512})
     for (int r : {1, 2, 4, 8})

@@ -59,6 +59,25 @@ def test_odd_graph_length_shapes_and_oracle(lbm, oracle_mod, prec):
             assert np.count_nonzero(bits(o.populations()) != bits(ref)) == 0, k
 
 
+@pytest.mark.parametrize("prec", ("float", "double"))
+@pytest.mark.parametrize("order", (0, 1, 2))
+def test_every_order_is_bit_identical(lbm, prec, order):
+    """The three ways of walking the work (grid-stride tiles, XCD-banded chunks, wave work-list
+    of 64-column segments) with several shapes: each domain equals its single-domain run."""
+    doms = lbm.random_domains(seed=13, n=20, nx_range=(33, 400), ny_range=(17, 150),
+                              radius_range=(2, 10))
+    b = lbm.Batch(doms, prec)
+    b.set_order(order, order != 1)
+    assert b.order() == (order, order != 1)
+    b.set_graph_steps(5)
+    for shape, steps in (((256, 2, 64), 7), ((512, 4, 9), 6), ((128, 1, 300), 5)):
+        b.set_shape(*shape)
+        b.run(steps)
+    for k, d in enumerate(doms):
+        ref = single(lbm, d, 18, prec)
+        assert np.count_nonzero(bits(b.populations(k)) != bits(ref)) == 0, (order, k)
+
+
 def test_autotune_keeps_results(lbm):
     doms = lbm.random_domains(seed=9, n=16, nx_range=(32, 300), ny_range=(32, 200))
     b = lbm.Batch(doms, "float")
