@@ -176,12 +176,29 @@ def form_adjoint(lbm, args):
         J, tf = a.forward()
         a.gradient()
         J2, dF, dtau, tg, fsteps = a.gradient()
+        sched = a.schedule_info()
         out[f"snapshots_{snaps}"] = {"forward_ms": round(tf * 1e3, 3), "gradient_ms": round(tg * 1e3, 3),
                                      "cost_vs_forward": round(tg / tf, 3),
-                                     "recomputed_forward_steps": fsteps, "J": J, "dJ_dtau": dtau}
+                                     "forward_steps": fsteps, "schedule": sched["kind"],
+                                     "stored_states": sched["slots"], "levels": sched["levels"],
+                                     "launches": sched["launches"], "J": J, "dJ_dtau": dtau}
         a.close()
         log(f"adjoint snapshots={snaps}: forward {tf * 1e3:.2f} ms, gradient {tg * 1e3:.2f} ms "
-            f"(x{tg / tf:.2f})")
+            f"(x{tg / tf:.2f}), {sched}")
+    # the binomial schedule at the sqrt(T) budget, for comparison
+    a = lbm.Adjoint(nx, ny, 0.7, solid, T, (384, 48, 16, 32), avg, 45, "float", "binomial")
+    a.set_force_field(F)
+    a.forward()
+    J, tf = a.forward()
+    a.gradient()
+    _, _, _, tg, fsteps = a.gradient()
+    sched = a.schedule_info()
+    out["snapshots_45_binomial"] = {"forward_ms": round(tf * 1e3, 3), "gradient_ms": round(tg * 1e3, 3),
+                                    "cost_vs_forward": round(tg / tf, 3), "forward_steps": fsteps,
+                                    "schedule": sched["kind"], "stored_states": sched["slots"],
+                                    "launches": sched["launches"]}
+    a.close()
+    log(f"adjoint binomial snapshots=45: gradient {tg * 1e3:.2f} ms (x{tg / tf:.2f})")
     return out
 
 

@@ -90,6 +90,17 @@ int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resou
 int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
                            int* tile_order);
 
+/* No reference counterpart: temporal blocking. With steps_per_launch = 2 (the default)
+ * lbm_run and lbm_timed_run advance pairs of steps in one launch of a two-step kernel wherever
+ * it applies (periodic domain, uniform force, float or double with the nine planes < 4 GiB):
+ * every population is read from HBM and written once per two steps. The results are bit-identical
+ * to single steps (same operations per cell). block_x (64..512, multiple of 64) and rows set its
+ * tile (0 keeps the current value); lbm_step always advances one step. in_use reports whether
+ * the current configuration takes the two-step path. */
+int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows);
+int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
+                             int* in_use);
+
 /* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */
 int lbm_step(lbm_solver* s);
 /* SolverV3::run(steps) — solver_v3.cu:280-283: steps, then synchronise. */

@@ -71,6 +71,8 @@ def _declare(L):
         "lbm_set_force_field": ([vp, vp], i),
         "lbm_set_tau": ([vp, d], i),
         "lbm_set_block": ([vp, i, i], i),
+        "lbm_set_steps_per_launch": ([vp, i, i, i], i),
+        "lbm_get_steps_per_launch": ([vp, ip, ip, ip, ip], i),
         "lbm_get_block": ([vp, ip, ip], i),
         "lbm_set_kernel_options": ([vp, i, i, i], i),
         "lbm_get_kernel_options": ([vp, ip, ip, ip], i),
@@ -365,6 +367,19 @@ class Solver:
 
     def set_block(self, block_x, rows_per_thread):
         check(lib().lbm_set_block(self._h, int(block_x), int(rows_per_thread)))
+
+    def set_steps_per_launch(self, steps_per_launch: int, block_x: int = 0, rows: int = 0):
+        """1: one step per launch; 2: run()/timed_run() advance pairs of steps in one launch of
+        the two-step kernel where it applies (same bits)."""
+        check(lib().lbm_set_steps_per_launch(self._h, int(steps_per_launch), int(block_x),
+                                             int(rows)))
+
+    def steps_per_launch(self) -> dict:
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().lbm_get_steps_per_launch(self._h, C.byref(a), C.byref(b), C.byref(c),
+                                             C.byref(d)))
+        return {"steps_per_launch": a.value, "block_x": b.value, "rows": c.value,
+                "in_use": bool(d.value)}
 
     def set_kernel_options(self, streaming_stores: bool, single_resource: bool,
                            tile_order: int = 0):

@@ -153,6 +153,14 @@ template <class S>
 hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,
                            int w, int h, float inv_scale, double* out, hipStream_t stream);
 
+// Two fused steps per launch (step2.hip): periodic domain, uniform force, nine planes < 4 GiB;
+// p.rows_per_tile = output rows per tile, block_x (64..512, a multiple of 64) columns per tile,
+// p.tiles_x = ceil(nx / block_x). Bit-identical to two launches of launch_step.
+template <class T>
+hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream);
+// LDS bytes per workgroup of launch_step2.
+size_t step2_lds_bytes(int block_x, int rows, size_t elem);
+
 // Strip ring closed on itself: ghost rows 0 and nyl + 1 of the state g (plane stride N, rows
 // of nx values) refreshed from the owned boundary rows, one launch.
 template <class T>

@@ -155,7 +155,7 @@ int lbm_step(lbm_solver* s) {
 int lbm_run(lbm_solver* s, int steps) {
   LBM_TRY(check_handle(s));
   if (steps < 0) return fail(LBM_EINVAL, "steps must be >= 0");
-  for (int k = 0; k < steps; ++k) LBM_TRY(s->step());
+  LBM_TRY(s->advance(steps));
   LBM_HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
@@ -164,7 +164,7 @@ int lbm_timed_run(lbm_solver* s, int steps, double* seconds) {
   LBM_TRY(check_handle(s));
   if (steps < 0) return fail(LBM_EINVAL, "steps must be >= 0");
   LBM_HIP_TRY(hipEventRecord(s->ev0, s->stream));
-  for (int k = 0; k < steps; ++k) LBM_TRY(s->step());
+  LBM_TRY(s->advance(steps));
   LBM_HIP_TRY(hipEventRecord(s->ev1, s->stream));
   LBM_HIP_TRY(hipEventSynchronize(s->ev1));
   float ms = 0.0f;
@@ -228,6 +228,29 @@ int lbm_set_state(lbm_solver* s, const void* in) {
   return LBM_OK;
 }
 
+int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows) {
+  LBM_TRY(check_handle(s));
+  if (steps_per_launch != 1 && steps_per_launch != 2)
+    return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
+  if (block_x != 0 && (block_x < 64 || block_x > 512 || block_x % 64))
+    return fail(LBM_EINVAL, "block_x must be 0 or a multiple of 64 in 64..512");
+  if (rows < 0) return fail(LBM_EINVAL, "rows must be >= 0");
+  s->steps_per_launch = steps_per_launch;
+  if (block_x) s->block2_x = block_x;
+  if (rows) s->rows2 = rows;
+  return LBM_OK;
+}
+
+int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
+                             int* in_use) {
+  LBM_TRY(check_handle(s));
+  if (steps_per_launch) *steps_per_launch = s->steps_per_launch;
+  if (block_x) *block_x = s->block2_x;
+  if (rows) *rows = s->rows2;
+  if (in_use) *in_use = s->two_step_eligible() ? 1 : 0;
+  return LBM_OK;
+}
+
 int lbm_threads(const lbm_solver*) { return 0; }
 
 int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
@@ -245,12 +268,12 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
                   "\"gpu\": \"%s\", \"arch\": \"%s\", \"hip_runtime\": %d, \"hip_driver\": %d, "
                   "\"block\": \"%dx%d\", \"variant\": \"soa_pull_buffer\", \"storage\": \"%s\", "
                   "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\", "
-                  "\"plane_stride\": %zu",
+                  "\"plane_stride\": %zu, \"steps_per_launch\": %d, \"two_step_tile\": \"%dx%d\"",
                   prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
                   s->xcd_bands == 1 ? "xcd_bands" : (s->xcd_bands == 2 ? "xcd_rows" : "round_robin"),
-                  s->S);
+                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->rows2);
   return LBM_OK;
 }
 

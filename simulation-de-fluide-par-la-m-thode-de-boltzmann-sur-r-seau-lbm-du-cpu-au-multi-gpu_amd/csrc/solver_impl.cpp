@@ -177,6 +177,9 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
     store_nt = 1;
     xcd_bands = prec == LBM_DOUBLE ? 1 : 0;
   }
+  // two-step kernel: 512-column tiles in float; 256 in double (its LDS ring doubles)
+  block2_x = prec == LBM_DOUBLE ? 256 : 512;
+  rows2 = 64;
   LBM_TRY(parse_block_env(&block_x, &rows));
   LBM_HIP_TRY(hipStreamCreate(&stream));
   LBM_HIP_TRY(hipEventCreate(&ev0));
@@ -326,6 +329,38 @@ int SolverState::step() {
   if (e != hipSuccess) return fail(LBM_EHIP, std::string("step launch: ") + hipGetErrorString(e));
   std::swap(g, gn);
   ++step_count;
+  return LBM_OK;
+}
+
+bool SolverState::two_step_eligible() const {
+  if (steps_per_launch != 2 || open_x || force || prec == LBM_FP16) return false;
+  if (kQ * S * storage_size(prec) >= (size_t(1) << 32)) return false;
+  return step2_lds_bytes(block2_x, rows2, storage_size(prec)) <= 64 * 1024;
+}
+
+int SolverState::advance(int n) {
+  if (n >= 2 && two_step_eligible()) {
+    hipError_t e = hipSuccess;
+    for (; n >= 2 && e == hipSuccess; n -= 2) {
+      if (prec == LBM_FLOAT) {
+        StepParams<float> p = make_step_params<float>(*this);
+        p.rows_per_tile = rows2;
+        p.tiles_x = (nx + block2_x - 1) / block2_x;
+        e = launch_step2(p, block2_x, stream);
+      } else {
+        StepParams<double> p = make_step_params<double>(*this);
+        p.rows_per_tile = rows2;
+        p.tiles_x = (nx + block2_x - 1) / block2_x;
+        e = launch_step2(p, block2_x, stream);
+      }
+      if (e == hipSuccess) {
+        std::swap(g, gn);
+        step_count += 2;
+      }
+    }
+    if (e != hipSuccess) return fail(LBM_EHIP, std::string("two-step launch: ") + hipGetErrorString(e));
+  }
+  for (; n > 0; --n) LBM_TRY(step());
   return LBM_OK;
 }
 

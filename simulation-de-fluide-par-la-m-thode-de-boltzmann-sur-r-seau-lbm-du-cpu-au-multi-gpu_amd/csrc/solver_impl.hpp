@@ -23,6 +23,8 @@ struct SolverState {
     const int tiles_y = (ny + rows_per_tile - 1) / rows_per_tile;
     return xcd_bands == 2 && tiles_y % 8 != 0 ? 0 : xcd_bands;
   }
+  int steps_per_launch = 2;     // run(n) with two fused steps per launch where eligible
+  int block2_x = 512, rows2 = 64;  // tile of the two-step kernel (columns, rows)
   double fp16_scale = 4096.0;   // stored half = g * scale (LBM_FP16)
   uint32_t step_count = 0;      // steps taken (seeds the FP16 stochastic rounding)
 
@@ -54,6 +56,10 @@ struct SolverState {
   int set_tau(double tau);
   int set_fp16_scale(double scale);
   int step();
+  // n steps: pairs of steps in one launch of the two-step kernel where eligible (periodic,
+  // uniform force, float or double with the nine planes < 4 GiB), the rest single steps.
+  int advance(int n);
+  bool two_step_eligible() const;
   int populations(void* out);
   int populations_at(int x, int y, void* out9);
   int set_populations(const void* in);

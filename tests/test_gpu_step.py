@@ -276,3 +276,49 @@ def test_tile_orders_give_identical_bits(lbm, oracle_mod, prec, order):
         ref.init_equilibrium(1.0, 0.02, 0.01)
         ref.run(120)
         assert_bitwise(runs[1], ref.populations(), f"tile order {order} vs oracle")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("grid,tile", [((53, 31), (512, 64)), ((53, 31), (64, 1)),
+                                       ((200, 70), (64, 7)), ((640, 416), (512, 64)),
+                                       ((640, 416), (128, 3)), ((1000, 300), (256, 16))])
+def test_two_step_launches_match_oracle(lbm, oracle_mod, prec, grid, tile):
+    """Temporal blocking: run(n) advances pairs of steps in one launch (tiles of block_x
+    columns by rows rows, halo columns and rows recomputed), odd n ends with a single step.
+    Walls, an obstacle touching a tile edge, an arbitrary state and a uniform force; bit for bit
+    against the oracle after 1, 2, 5 and 8 steps."""
+    nx, ny = grid
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[ny // 3: ny // 3 + 9, min(nx - 12, 60): min(nx - 12, 60) + 7] = 1
+    solid[ny // 2, :: 7] = 1
+    solid = solid.reshape(-1)
+    rng = np.random.default_rng(nx * 7 + ny)
+    dt = np.float32 if prec == "float" else np.float64
+    S = (rng.standard_normal(nx * ny * 9) * 1e-3).astype(dt)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    if prec == "double":  # its LDS ring holds twice the bytes: at most 256 columns
+        tile = (min(tile[0], 256), tile[1])
+    s = lbm.Solver(nx, ny, 0.73, 2e-5, -7e-6, solid, prec)
+    s.set_steps_per_launch(2, *tile)
+    assert s.steps_per_launch()["in_use"]
+    o = oracle_mod.Oracle(nx, ny, 0.73, 2e-5, -7e-6, solid, prec, "v0")
+    for x in (s, o):
+        x.set_populations(S)
+    done = 0
+    for n in (1, 1, 3, 3):
+        s.run(n)
+        o.run(n)
+        done += n
+        assert_bitwise(s.populations(), o.populations(), f"{grid} tile {tile} after {done} steps")
+
+
+def test_two_step_path_is_the_default_where_it_applies(lbm):
+    s = lbm.Solver(256, 64, 0.8, 1e-5, 0.0, lbm.make_channel(256, 64), "float")
+    assert s.steps_per_launch()["in_use"]
+    s.set_inflow_outflow(0.01)
+    assert not s.steps_per_launch()["in_use"]   # open x: single steps
+    s2 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "fp16")
+    assert not s2.steps_per_launch()["in_use"]  # FP16 storage: single steps
+    s3 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "double")
+    s3.set_force_field(np.zeros(2 * 256 * 64))
+    assert not s3.steps_per_launch()["in_use"]  # per-cell force field: single steps
