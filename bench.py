@@ -5,17 +5,20 @@ force-driven channel with walls at y = 0 and y = ny - 1, periodic in x, tau = 0.
 rest initial state; `--steps` timed steps (default 200) after `--warmup` untimed ones.
 MLUPS = nx * ny * steps / t / 1e6, counting every cell (reference src/common/driver.hpp:120).
 Timing: HIP events recorded on the solver's stream around the timed steps (lbm_timed_run).
+The solver's run() advances pairs of steps in one launch of the two-step kernel (bit-identical
+to two single steps); an odd --steps ends with one single-step launch.
 
 Besides the headline value the line carries:
-  roofline      72 algorithmic bytes per cell update (9 reads + 9 writes of fp32) x cells per
-                launch / average launch time, against the 8 TB/s HBM peak; `traffic` is the
-                rocprofv3-measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the
-                copy-kernel calibration), read from the committed profile summary;
+  roofline      of the two-step kernel: 72 algorithmic bytes per cell per launch (each of the 9
+                fp32 populations read once and written once for two updates) x cells / average
+                launch time, against the 8 TB/s HBM peak; `traffic` is the rocprofv3-measured
+                HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+                calibration), read from the committed profile summary;
   cpu_baseline  the reference's own v1 OpenMP solver compiled from its sources (oracle/_ref),
                 or the oracle's OpenMP port when that is absent, timed on this host on a
                 bounded sample of the same channel;
-  forms         the other measured forms (copy ceiling, fp64, FP16 storage, batched domains,
-                adjoint cost) when --forms is given.
+  forms         the other measured forms (one step per launch, copy ceiling, fp64, FP16
+                storage, batched domains, strip share, adjoint cost, cylinder wake).
 """
 from __future__ import annotations
 
@@ -34,7 +37,7 @@ import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"float": 72, "double": 144, "fp16": 36}
-PROFILE_SUMMARY = ROOT / "profiles" / "step_fp32_8192x8192.md"
+PROFILE_SUMMARY = ROOT / "profiles" / "step2_fp32_8192x8192.md"
 
 
 def log(msg: str) -> None:
@@ -42,17 +45,18 @@ def log(msg: str) -> None:
 
 
 def traffic_from_profile():
-    """HBM bytes per launch of the plain fp32 step, from the committed rocprofv3 summary."""
+    """HBM bytes per launch of the fp32 two-step kernel, from the committed rocprofv3 summary."""
     if not PROFILE_SUMMARY.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", PROFILE_SUMMARY.read_text())
     return float(m.group(1)) if m else None
 
 
-def run_plain(lbm, nx, ny, steps, warmup, precision="float", block=None):
+def run_plain(lbm, nx, ny, steps, warmup, precision="float", block=None, steps_per_launch=2):
     s = lbm.Solver(nx, ny, 0.6, 1e-6, 0.0, lbm.make_channel(nx, ny), precision)
     if block:
         s.set_block(*block)
+    s.set_steps_per_launch(steps_per_launch)
     s.init_equilibrium(1.0, 0.0, 0.0)
     s.run(warmup)
     t = s.timed_run(steps)
@@ -81,6 +85,25 @@ def cpu_baseline(nx, ny, steps=10, warmup=2):
         kind = "port"
     return {"value": nx * ny * steps / sec / 1e6, "unit": "MLUPS", "cores": used or threads,
             "kind": kind, "sample": sample, "seconds": sec}
+
+
+def form_single_step(lbm, args):
+    """The same workload with one step per launch (step_kernel): 72 B (fp32) or 144 B (fp64)
+    per cell update; the fp32 figures at the top level, fp64 under "double"."""
+    out = {}
+    for prec in ("float", "double"):
+        t, info, _ = run_plain(lbm, args.nx, args.ny, args.steps, args.warmup, prec, None, 1)
+        mlups = args.nx * args.ny * args.steps / t / 1e6
+        gbs = mlups * 1e6 * BYTES_PER_CELL[prec] / 1e9
+        log(f"{prec} single step per launch: {mlups:.0f} MLUPS, {gbs:.0f} GB/s")
+        r = {"mlups": round(mlups, 1), "timed_steps": args.steps,
+             "hbm_gbs_algorithmic": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+             "solver": json.loads("{" + info + "}")}
+        if prec == "float":
+            out.update(r)
+        else:
+            out["double"] = r
+    return out
 
 
 def form_copy_ceiling(lbm, args):
@@ -113,9 +136,12 @@ def form_precisions(lbm, args):
         s.run(20)
         t = s.timed_run(steps - 20)
         mlups = nx * ny * (steps - 20) / t / 1e6
+        per_launch = 2 if s.steps_per_launch()["in_use"] else 1
         ux[prec] = s.macroscopic()[1][fluid].astype(np.float64)
         out[f"plain_{prec}"] = {"mlups": round(mlups, 1), "timed_steps": steps - 20,
-                                "hbm_gbs_algorithmic": round(mlups * 1e6 * BYTES_PER_CELL[prec] / 1e9, 1),
+                                "steps_per_launch": per_launch,
+                                "hbm_gbs_algorithmic": round(
+                                    mlups * 1e6 * BYTES_PER_CELL[prec] / per_launch / 1e9, 1),
                                 "mass_drift_fp64_reduction": (s.total_mass() - m0) / m0}
         s.close()
         log(f"{prec}: {mlups:.0f} MLUPS")
@@ -218,7 +244,8 @@ def form_cylinder(lbm, args):
     return out
 
 
-FORMS = {"copy_ceiling": form_copy_ceiling, "precisions": form_precisions,
+FORMS = {"single_step": form_single_step, "copy_ceiling": form_copy_ceiling,
+         "precisions": form_precisions,
          "batched": form_batched, "strip": form_strip, "adjoint": form_adjoint,
          "cylinder": form_cylinder}
 
@@ -264,12 +291,16 @@ def main():
     log(f"device: {lbm.device_info()}")
     block = tuple(int(v) for v in args.block.split("x")) if args.block else None
     t, info, mass = run_plain(lbm, args.nx, args.ny, args.steps, args.warmup, "float", block)
+    solver = json.loads("{" + info + "}")
     cells = args.nx * args.ny
     mlups = cells * args.steps / t / 1e6
-    launch_s = t / args.steps
+    per_launch = solver.get("steps_per_launch", 1)  # 2: the two-step kernel
+    launches = args.steps / per_launch
+    launch_s = t / launches
+    # each launch reads and writes every population once, whether it advances one or two steps
     achieved = BYTES_PER_CELL["float"] * cells / launch_s / 1e9
-    log(f"plain fp32 {args.nx}x{args.ny}: {mlups:.1f} MLUPS, {launch_s * 1e3:.4f} ms/step, "
-        f"{achieved:.0f} GB/s algorithmic; {info}")
+    log(f"plain fp32 {args.nx}x{args.ny}: {mlups:.1f} MLUPS, {t / args.steps * 1e3:.4f} ms/step, "
+        f"{per_launch} steps per launch, {achieved:.0f} GB/s algorithmic per launch; {info}")
 
     line = {
         "metric": "MLUPS & % HBM roofline, 8192² fp32 plain vs fp16-store/batched; adjoint cost",
@@ -278,7 +309,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(launch_s * 1e3, 5),
+        "ms_per_step": round(t / args.steps * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -288,20 +319,33 @@ def main():
                                f"bounce-back, f-w storage), {args.nx}x{args.ny} walled channel, "
                                f"tau 0.6, gx 1e-6, rest init",
                    "nx": args.nx, "ny": args.ny, "parallelism": "single GPU",
-                   "solver": json.loads("{" + info + "}")},
+                   "solver": solver},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile()},
+                     "traffic": traffic_from_profile(),
+                     "kernel": "step2_kernel<float>" if per_launch == 2 else "step_kernel<float>",
+                     "updates_per_launch": per_launch,
+                     "bytes_per_launch": BYTES_PER_CELL["float"] * cells,
+                     "launch_ms": round(launch_s * 1e3, 5)},
     }
     if not args.no_forms:
         line["forms"] = forms(lbm, args)
-        cc = line["forms"].get("copy_ceiling", {})
+        f = line["forms"]
+        cc = f.get("copy_ceiling", {})
         if cc.get("mlups_ceiling_fp32"):
-            line["forms"]["plain_float_frac_of_copy_ceiling"] = round(mlups / cc["mlups_ceiling_fp32"], 4)
-            line["forms"]["plain_float_frac_of_reference_method_copy"] = round(
-                achieved / cc["reference_method_copy_gbs"], 4)
-            line["forms"]["plain_float_frac_of_nine_array_copy"] = round(
-                achieved / cc["nine_array_copy_gbs"], 4)
+            # MLUPS against the copy ceiling / 72 B per update; >1 once a launch advances two steps
+            f["plain_float_mlups_vs_copy_ceiling"] = round(mlups / cc["mlups_ceiling_fp32"], 4)
+            f["plain_float_launch_bytes_frac_of_copy_ceiling"] = round(
+                achieved / max(cc["copy_kernel_gbs"], cc["memcpy_dtod_gbs"]), 4)
+            ss = f.get("single_step", {})
+            if ss.get("mlups"):
+                f["single_step_frac_of_copy_ceiling"] = round(ss["mlups"] / cc["mlups_ceiling_fp32"], 4)
+                f["single_step_frac_of_nine_array_copy"] = round(
+                    ss["hbm_gbs_algorithmic"] / cc["nine_array_copy_gbs"], 4)
+        pr, ss = f.get("precisions", {}), f.get("single_step", {})
+        if pr.get("plain_fp16", {}).get("mlups") and ss.get("mlups"):
+            f["fp16_vs_fp32_single_step"] = round(pr["plain_fp16"]["mlups"] / ss["mlups"], 3)
+            f["fp16_vs_fp32_default"] = round(pr["plain_fp16"]["mlups"] / mlups, 3)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(args.nx, args.ny, steps=args.cpu_steps)

@@ -92,11 +92,14 @@ int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* sing
 
 /* No reference counterpart: temporal blocking. With steps_per_launch = 2 (the default)
  * lbm_run and lbm_timed_run advance pairs of steps in one launch of a two-step kernel wherever
- * it applies (periodic domain, uniform force, float or double with the nine planes < 4 GiB):
- * every population is read from HBM and written once per two steps. The results are bit-identical
- * to single steps (same operations per cell). block_x (64..512, multiple of 64) and rows set its
- * tile (0 keeps the current value); lbm_step always advances one step. in_use reports whether
- * the current configuration takes the two-step path. */
+ * it applies (periodic domain, uniform force, float or double): every population is read from
+ * HBM and written once per two steps. The results are bit-identical to single steps (same
+ * operations per cell). block_x (64..512, multiple of 64; 0 keeps the current value) and rows
+ * set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64, 48, 32, 24, 16
+ * rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps the current
+ * setting. lbm_step always advances one
+ * step. lbm_get_steps_per_launch reports the height in use and in_use whether the current
+ * configuration takes the two-step path. */
 int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows);
 int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
                              int* in_use);

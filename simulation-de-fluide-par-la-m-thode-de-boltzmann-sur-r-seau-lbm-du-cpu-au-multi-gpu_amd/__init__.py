@@ -370,7 +370,8 @@ class Solver:
 
     def set_steps_per_launch(self, steps_per_launch: int, block_x: int = 0, rows: int = 0):
         """1: one step per launch; 2: run()/timed_run() advance pairs of steps in one launch of
-        the two-step kernel where it applies (same bits)."""
+        the two-step kernel where it applies (same bits). block_x: tile columns (0 keeps);
+        rows: tile rows, -1 automatic (up to 64, filling the GPU evenly), 0 keeps."""
         check(lib().lbm_set_steps_per_launch(self._h, int(steps_per_launch), int(block_x),
                                              int(rows)))
 

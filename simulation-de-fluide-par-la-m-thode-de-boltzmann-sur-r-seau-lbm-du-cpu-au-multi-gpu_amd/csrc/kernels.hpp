@@ -153,13 +153,23 @@ template <class S>
 hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,
                            int w, int h, float inv_scale, double* out, hipStream_t stream);
 
-// Two fused steps per launch (step2.hip): periodic domain, uniform force, nine planes < 4 GiB;
-// p.rows_per_tile = output rows per tile, block_x (64..512, a multiple of 64) columns per tile,
-// p.tiles_x = ceil(nx / block_x). Bit-identical to two launches of launch_step.
+// Values every population plane holds past its nx * ny cells (plane stride N >= nx * ny +
+// kPlaneTail): launch_step2 stores the results of solid cells there instead of skipping them.
+constexpr size_t kPlaneTail = 64;
+
+// Two fused steps per launch (step2.hip): periodic domain, uniform force, planes < 4 GiB with
+// the kPlaneTail tail; p.rows_per_tile = output rows per tile, block_x (64..512, a multiple of
+// 64) columns per tile, p.tiles_x = ceil(nx / block_x). Bit-identical to two launches of
+// launch_step (solid cells keep their values in both).
 template <class T>
 hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream);
-// LDS bytes per workgroup of launch_step2.
-size_t step2_lds_bytes(int block_x, int rows, size_t elem);
+// LDS bytes per workgroup of launch_step2 (independent of the tile height).
+size_t step2_lds_bytes(int block_x, size_t elem);
+// Tile height for launch_step2: the tallest of 64, 48, 32, 24, 16 rows whose tile count fills
+// the resident workgroup slots (occupancy of the instance p selects, on the current device)
+// evenly, else the best-filling one.
+template <class T>
+int step2_tile_rows(const StepParams<T>& p, int block_x);
 
 // Strip ring closed on itself: ghost rows 0 and nyl + 1 of the state g (plane stride N, rows
 // of nx values) refreshed from the owned boundary rows, one launch.

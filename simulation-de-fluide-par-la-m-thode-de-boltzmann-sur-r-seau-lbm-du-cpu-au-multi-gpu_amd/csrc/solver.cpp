@@ -234,9 +234,12 @@ int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, i
     return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
   if (block_x != 0 && (block_x < 64 || block_x > 512 || block_x % 64))
     return fail(LBM_EINVAL, "block_x must be 0 or a multiple of 64 in 64..512");
-  if (rows < 0) return fail(LBM_EINVAL, "rows must be >= 0");
+  if (rows < -1) return fail(LBM_EINVAL, "rows must be -1 (automatic), 0 (keep) or positive");
   s->steps_per_launch = steps_per_launch;
-  if (block_x) s->block2_x = block_x;
+  if (block_x && block_x != s->block2_x) {
+    s->block2_x = block_x;
+    s->rows2_auto = 0;
+  }
   if (rows) s->rows2 = rows;
   return LBM_OK;
 }
@@ -246,7 +249,7 @@ int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* bl
   LBM_TRY(check_handle(s));
   if (steps_per_launch) *steps_per_launch = s->steps_per_launch;
   if (block_x) *block_x = s->block2_x;
-  if (rows) *rows = s->rows2;
+  if (rows) *rows = s->two_step_rows();
   if (in_use) *in_use = s->two_step_eligible() ? 1 : 0;
   return LBM_OK;
 }
@@ -273,7 +276,7 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
                   s->xcd_bands == 1 ? "xcd_bands" : (s->xcd_bands == 2 ? "xcd_rows" : "round_robin"),
-                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->rows2);
+                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->two_step_rows());
   return LBM_OK;
 }
 

@@ -125,7 +125,8 @@ uint16_t to_half(float f) {
 }  // namespace
 
 size_t plane_stride(size_t N, size_t elem_bytes) {
-  if (N * elem_bytes < (size_t(1) << 20)) return N;
+  // At least kPlaneTail values past the last cell (the two-step kernel's discard slots).
+  if (N * elem_bytes < (size_t(1) << 20)) return (N + 63) / 64 * 64 + kPlaneTail;
   const size_t q = 65536 / elem_bytes;
   return (N + q - 1) / q * q + 4352 / elem_bytes;
 }
@@ -177,9 +178,11 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
     store_nt = 1;
     xcd_bands = prec == LBM_DOUBLE ? 1 : 0;
   }
-  // two-step kernel: 512-column tiles in float; 256 in double (its LDS ring doubles)
+  // two-step kernel: 512-column tiles in float; 256 in double (its LDS ring doubles; 512
+  // columns exceed the LDS of a workgroup). 8192^2: fp32 512 x 64 146.3k MLUPS (256 x 64
+  // 128.5k), fp64 256 x 64 69.6k (128 x 64 63.7k); the height comes from step2_tile_rows().
   block2_x = prec == LBM_DOUBLE ? 256 : 512;
-  rows2 = 64;
+  rows2 = -1;
   LBM_TRY(parse_block_env(&block_x, &rows));
   LBM_HIP_TRY(hipStreamCreate(&stream));
   LBM_HIP_TRY(hipEventCreate(&ev0));
@@ -192,7 +195,8 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
   if (hipMalloc(&links, cells() * sizeof(uint16_t)) != hipSuccess)
     return fail(LBM_ENOMEM, "device allocation of the link codes failed");
   nseg = (nx + 63) / 64;
-  if (hipMalloc(&segflags, static_cast<size_t>(ny) * nseg) != hipSuccess)
+  // whole dwords: the two-step kernel reads a flag with a scalar load of its aligned dword
+  if (hipMalloc(&segflags, (static_cast<size_t>(ny) * nseg + 3) & ~size_t(3)) != hipSuccess)
     return fail(LBM_ENOMEM, "device allocation of the segment flags failed");
   if (hipMalloc(&red, sizeof(double) * (4 * kReduceBlocks + 4)) != hipSuccess)
     return fail(LBM_ENOMEM, "device allocation of the reduction buffer failed");
@@ -334,23 +338,38 @@ int SolverState::step() {
 
 bool SolverState::two_step_eligible() const {
   if (steps_per_launch != 2 || open_x || force || prec == LBM_FP16) return false;
-  if (kQ * S * storage_size(prec) >= (size_t(1) << 32)) return false;
-  return step2_lds_bytes(block2_x, rows2, storage_size(prec)) <= 64 * 1024;
+  if (S * storage_size(prec) >= (size_t(1) << 32)) return false;
+  return step2_lds_bytes(block2_x, storage_size(prec)) <= 64 * 1024;
+}
+
+int SolverState::two_step_rows() const {
+  if (rows2 > 0) return rows2;
+  if (!rows2_auto) {
+    if (prec == LBM_DOUBLE) {
+      rows2_auto = step2_tile_rows(make_step_params<double>(*this), block2_x);
+    } else {
+      rows2_auto = step2_tile_rows(make_step_params<float>(*this), block2_x);
+    }
+  }
+  return rows2_auto;
 }
 
 int SolverState::advance(int n) {
   if (n >= 2 && two_step_eligible()) {
     hipError_t e = hipSuccess;
+    const int rows_tile = two_step_rows();
     for (; n >= 2 && e == hipSuccess; n -= 2) {
       if (prec == LBM_FLOAT) {
         StepParams<float> p = make_step_params<float>(*this);
-        p.rows_per_tile = rows2;
+        p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
+        p.xcd_bands = two_step_order;
         e = launch_step2(p, block2_x, stream);
       } else {
         StepParams<double> p = make_step_params<double>(*this);
-        p.rows_per_tile = rows2;
+        p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
+        p.xcd_bands = two_step_order;
         e = launch_step2(p, block2_x, stream);
       }
       if (e == hipSuccess) {

@@ -24,7 +24,13 @@ struct SolverState {
     return xcd_bands == 2 && tiles_y % 8 != 0 ? 0 : xcd_bands;
   }
   int steps_per_launch = 2;     // run(n) with two fused steps per launch where eligible
-  int block2_x = 512, rows2 = 64;  // tile of the two-step kernel (columns, rows)
+  int block2_x = 512;  // tile columns of the two-step kernel
+  int rows2 = -1;      // tile rows; -1: chosen by step2_tile_rows() (64 on large grids)
+  mutable int rows2_auto = 0;  // that height, once computed (0: not yet)
+  // Tile order of the two-step kernel: XCD bands (a tile's neighbours above, below and beside it
+  // mostly run on its own XCD, whose L2 then serves the halo rows and columns): 135.2k MLUPS at
+  // 8192^2 fp32 against 132.0k round-robin and 131.6k XCD rows.
+  int two_step_order = 1;
   double fp16_scale = 4096.0;   // stored half = g * scale (LBM_FP16)
   uint32_t step_count = 0;      // steps taken (seeds the FP16 stochastic rounding)
 
@@ -39,7 +45,8 @@ struct SolverState {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-  // Population planes are S >= nx * ny values apart (plane_stride(): padded on large grids).
+  // Population planes are S >= nx * ny + kPlaneTail values apart (plane_stride(); large grids
+  // are padded further to spread the planes over the memory channels).
   size_t S = 0;
   size_t cells() const { return static_cast<size_t>(nx) * ny; }
   size_t state_bytes() const;  // 9 planes of S values of the storage type
@@ -60,6 +67,7 @@ struct SolverState {
   // uniform force, float or double with the nine planes < 4 GiB), the rest single steps.
   int advance(int n);
   bool two_step_eligible() const;
+  int two_step_rows() const;  // tile rows the two-step kernel uses
   int populations(void* out);
   int populations_at(int x, int y, void* out9);
   int set_populations(const void* in);

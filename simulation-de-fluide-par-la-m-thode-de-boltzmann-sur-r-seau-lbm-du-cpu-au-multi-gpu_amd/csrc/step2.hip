@@ -14,9 +14,19 @@
 //      thread's own registers (rows r - 2, r - 1, r), the others from the ring (columns x - 1,
 //      x + 1), bounce-back from the thread's own R_{t+1} values of row r - 1; and stores it.
 // Step 1 therefore runs on rows y0 - 1 .. y1 (one extra row on each side of the tile) and the
-// two columns just outside the tile (x0 - 1 and x0 + W) are computed once per row in a prologue
-// (only the three populations the tile pulls from each of them are kept, in LDS). One barrier
-// per row: with four ring slots a slot is rewritten two barriers after its last read.
+// two columns just outside the tile (x0 - 1 and x0 + W) are computed for 32 rows at a time
+// (only the three populations the tile pulls from each of them are kept). Each ring row holds
+// W + 2 columns: column 0 is the left halo column, 1 .. W the tile, W + 1 the right halo, so a
+// lane reads its x - 1 and x + 1 neighbours at fixed offsets, the tile edges included. One
+// barrier per row: with four ring slots a slot is rewritten two barriers after its last read.
+// The recomputed rows and columns are the overhead of the scheme (rows y0 - 1 and y1 and their
+// sources y0 - 2 and y1 + 1 are read by two tiles); step2_tile_rows() picks tiles of up to 64
+// rows whose count fills the resident workgroup slots evenly. Solid cells are never written (both population
+// buffers keep the values every upload gives them); their results go to discard slots past
+// each plane's last cell.
+#include <algorithm>
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "step_body.hpp"
 
@@ -24,7 +34,7 @@ namespace lbmhip {
 namespace {
 
 constexpr int kRingSlots = 4;
-// Planes published in the ring and their slot index: the ones neighbours pull (c_x != 0).
+// Planes published in the ring and their index there: the ones neighbours pull (c_x != 0).
 __host__ __device__ constexpr int ring_index(int plane) {
   return plane == 1 ? 0 : plane == 3 ? 1 : plane == 5 ? 2 : plane == 6 ? 3 : plane == 7 ? 4 : 5;
 }
@@ -33,137 +43,307 @@ constexpr int kRingPlanes = 6;
 // the right one (x0 + W) its c_x = -1 planes.
 constexpr int kLeftPlanes[3] = {1, 5, 8};
 constexpr int kRightPlanes[3] = {3, 6, 7};
+constexpr uint32_t kLinkBounce = 0x1feu;  // bits 1..8 of a link code
 
-// Step 1 at one cell (c, r) (both wrapped): the pull with its boundary corrections and the
-// collision, exactly as update_column. Returns the link code (bit 0: solid, then v is
-// meaningless).
-template <class T>
-__device__ __forceinline__ uint32_t first_step(const StepParams<T>& p, const PlanesOne& g, int c,
-                                               int r, T (&v)[kQ]) {
-  constexpr unsigned B = sizeof(T);
-  const int nx = p.nx, ny = p.ny;
+// Byte offsets of the source rows of step-1 row r (wrapped) and of the source columns of c.
+__device__ __forceinline__ Rows rows_of(int r, int nx, int ny, unsigned B) {
   const int rm = r == 0 ? ny - 1 : r - 1, rp = r == ny - 1 ? 0 : r + 1;
-  const Rows rows{static_cast<unsigned>(rm * nx) * B, static_cast<unsigned>(r * nx) * B,
-                  static_cast<unsigned>(rp * nx) * B};
-  const Cols cols{static_cast<unsigned>(c == 0 ? nx - 1 : c - 1) * B, static_cast<unsigned>(c) * B,
-                  static_cast<unsigned>(c == nx - 1 ? 0 : c + 1) * B};
-  pull_streamed<T>(g, rows, cols, v);
-  uint32_t link = 0;
-  if (p.segflags[static_cast<size_t>(r) * p.nseg + (c >> 6)]) {
-    link = p.links[static_cast<size_t>(r) * nx + c];
-    if (!(link & kLinkSolid)) apply_links<T, false>(link, g, rows, cols, static_cast<const T*>(nullptr), v);
-  }
-  if (!(link & kLinkSolid)) collide(v, p.fx, p.fy, p.omega, p.fp);
-  return link;
+  return Rows{static_cast<unsigned>(rm * nx) * B, static_cast<unsigned>(r * nx) * B,
+              static_cast<unsigned>(rp * nx) * B};
+}
+// The same for a row that is wave-uniform, stated so: the compiler treats values computed in
+// the (divergent) prologue loop as per-lane, and a per-lane scalar offset costs a loop per load.
+__device__ __forceinline__ Rows uniform_rows_of(int r, int nx, int ny, unsigned B) {
+  const Rows a = rows_of(r, nx, ny, B);
+  auto u = [](unsigned x) { return static_cast<unsigned>(__builtin_amdgcn_readfirstlane(x)); };
+  return Rows{u(a.m), u(a.c), u(a.p)};
+}
+__device__ __forceinline__ Cols cols_of(int c, int nx, unsigned B) {
+  return Cols{static_cast<unsigned>(c == 0 ? nx - 1 : c - 1) * B, static_cast<unsigned>(c) * B,
+              static_cast<unsigned>(c == nx - 1 ? 0 : c + 1) * B};
 }
 
-template <class T, int kStoreAux>
+// Segment flag at a wave-uniform byte index, by a scalar load of its aligned dword (the flag
+// array is allocated in whole dwords): the branch on it is a scalar branch, and the wait for
+// it does not wait for the wave's outstanding vector stores as a vector load's wait would.
+__device__ __forceinline__ uint32_t uniform_flag(const uint8_t* flags, size_t index) {
+  using ConstU32 = const __attribute__((address_space(4))) uint32_t*;
+  const size_t at = static_cast<size_t>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(index)));
+  const uint32_t w = *(ConstU32)(flags + (at & ~size_t(3)));
+  return (w >> ((at & 3u) * 8u)) & 0xffu;
+}
+
+// Step 1 at a halo-column cell (c, r) (both wrapped) whose row differs between the lanes of a
+// wave: the row offsets go into the per-lane offset (a per-lane scalar offset would make the
+// compiler loop over its values). The pull with its bounce-back corrections and the collision,
+// exactly as update_column.
+template <class T, class PS>
+__device__ __forceinline__ void halo_step(const StepParams<T>& p, const PS& g, int c, int r,
+                                          T (&v)[kQ]) {
+  constexpr unsigned B = sizeof(T);
+  const Rows rows = rows_of(r, p.nx, p.ny, B);
+  const Cols cols = cols_of(c, p.nx, B);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) v[i] = g.template load<T>(i, cols.of(kCx[i]) + rows.of(kCy[i]), 0u);
+  uint32_t link = 0;
+  if (p.segflags[static_cast<size_t>(r) * p.nseg + (c >> 6)]) {
+    link = p.links[static_cast<size_t>(r) * p.nx + c];
+    if (!(link & kLinkSolid)) {
+#pragma unroll
+      for (int i = 1; i < kQ; ++i)
+        if (link & (1u << i)) v[i] = g.template load<T>(kOpp[i], cols.c + rows.c, 0u);
+    }
+  }
+  if (!(link & kLinkSolid)) collide(v, p.fx, p.fy, p.omega, p.fp);
+}
+
+// Halo-column cells are computed for kHaloChunk step-1 rows at a time (the first chunk also
+// covers the two leading rows), so the LDS a workgroup needs does not grow with its tile height.
+constexpr int kHaloChunk = 32;
+constexpr int kHaloRows = kHaloChunk + 2;
+
+template <class T, class PS, int kStoreAux>
 __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int W = static_cast<int>(blockDim.x);
+  const int W = static_cast<int>(blockDim.x), WP = W + 2;
   const int H = p.rows_per_tile;
-  T* ring = reinterpret_cast<T*>(smem);                 // [kRingSlots][kRingPlanes][W]
-  T* halo = ring + kRingSlots * kRingPlanes * W;        // [2][H + 2][3]
+  T* ring = reinterpret_cast<T*>(smem);                  // [kRingSlots][kRingPlanes][W + 2]
+  T* halo = ring + kRingSlots * kRingPlanes * WP;        // [2][kHaloRows][3]
   const int t = static_cast<int>(threadIdx.x);
   const int nx = p.nx, ny = p.ny;
-  const int tx = static_cast<int>(blockIdx.x % static_cast<unsigned>(p.tiles_x));
-  const int ty = static_cast<int>(blockIdx.x / static_cast<unsigned>(p.tiles_x));
+  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
+  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
+  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const int x0 = tx * W, wv = min(W, nx - x0);  // valid columns of this tile
   const int y0 = ty * H, y1 = min(y0 + H, ny);
   const int nr = y1 - y0 + 2;                   // step-1 rows y0 - 1 .. y1
-  PlanesOne g, gn;
+  PS g, gn;
   g.init(p.g, p.N);
   gn.init(p.gn, p.N);
   auto wrap_y = [&](int r) { return r < 0 ? r + ny : (r >= ny ? r - ny : r); };
 
-  // prologue: step 1 on the two halo columns for every step-1 row of the tile
-  for (int j = t; j < 2 * nr; j += W) {
-    const int side = j >= nr ? 1 : 0, k = j - side * nr;
-    const int c = side ? (x0 + wv == nx ? 0 : x0 + wv) : (x0 == 0 ? nx - 1 : x0 - 1);
-    T v[kQ];
-    first_step(p, g, c, wrap_y(y0 - 1 + k), v);
-    T* h = halo + (side * (H + 2) + k) * 3;
+  // Step 1 on the two halo columns for step-1 rows k0 .. k0 + cnt - 1 (cnt <= kHaloRows), into
+  // halo[side][k - k0]; then a barrier. A loop with a wave-uniform trip count: values the
+  // compiler shares with the main loop stay scalar.
+  const int cl = x0 == 0 ? nx - 1 : x0 - 1, cr = x0 + wv == nx ? 0 : x0 + wv;
+  auto halo_chunk = [&](const int k0, const int cnt) {
+    for (int j0 = 0; j0 < 2 * cnt; j0 += W) {
+      const int j = j0 + t;
+      if (j < 2 * cnt) {
+        const int side = j >= cnt ? 1 : 0, kk = j - side * cnt;
+        T v[kQ];
+        halo_step(p, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v);
+        T* h = halo + (side * kHaloRows + kk) * 3;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) h[q] = v[side ? kRightPlanes[q] : kLeftPlanes[q]];
-  }
-  __syncthreads();
+        for (int q = 0; q < 3; ++q) h[q] = v[side ? kRightPlanes[q] : kLeftPlanes[q]];
+      }
+    }
+    __syncthreads();
+  };
+  halo_chunk(0, min(kHaloRows, nr));
 
-  const bool active = t < wv;
-  const int c = active ? x0 + t : x0 + wv - 1;  // idle lanes shadow the last column
+  // Idle lanes (past the tile's last column) shadow the last column in everything, their ring
+  // column included: they compute its values and store them to its address, the same bits.
+  const int tt = t < wv ? t : wv - 1;
+  const int c = x0 + tt;
   constexpr unsigned B = sizeof(T);
+  const unsigned vo = static_cast<unsigned>(c) * B;
+  // A solid cell is not written: its step-2 result goes to a discard slot past the plane's
+  // last cell (a per-lane offset instead of a branch around the stores, see below).
+  const unsigned tail = static_cast<unsigned>(nx * ny) * B + static_cast<unsigned>(t & 63) * B;
+  const Cols cols = cols_of(c, nx, B);
   T prev[kQ];           // R_{t+1} at (c, r - 1)
   T prev2 = T(0);       // plane 2 of R_{t+1} at (c, r - 2)
   uint32_t prev_link = kLinkSolid;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
-  for (int k = 0; k < nr; ++k) {
+  T cur[kQ];            // raw pulls of the current step-1 row
+  pull_streamed<T>(g, uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cols, cur);
+
+  // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
+  // are issued first and nothing between them and their use branches around a vector memory
+  // operation (the bounce-back loads complete inside their own branch, solid cells store to
+  // the discard slots): one wait counter covers loads and stores, and a branch around either
+  // would make the wait for the pulls a wait for every store in flight.
+  auto row = [&](const int k, const int hk, const int slot, const bool two)
+                 __attribute__((always_inline)) {
     const int r = wrap_y(y0 - 1 + k);
+    const Rows rows = uniform_rows_of(r, nx, ny, B);
+    T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
+    pull_streamed<T>(g, uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), cols, nxt);
+    // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
+    // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
-    const uint32_t link = first_step(p, g, c, r, v);
-    T* slot = ring + (k & (kRingSlots - 1)) * kRingPlanes * W;
 #pragma unroll
-    for (int i : {1, 3, 5, 6, 7, 8}) slot[ring_index(i) * W + t] = v[i];
+    for (int i = 0; i < kQ; ++i) v[i] = cur[i];
+    uint32_t link = 0;
+    if (uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6))) {
+      link = p.links[static_cast<size_t>(r) * nx + c];
+      T own[kQ];
+#pragma unroll
+      for (int i = 1; i < kQ; ++i) own[i] = g.template load<T>(kOpp[i], cols.c, rows.c);
+#pragma unroll
+      for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? own[i] : v[i];
+    }
+    collide(v, p.fx, p.fy, p.omega, p.fp);
+    T* sl = ring + slot * kRingPlanes * WP;
+#pragma unroll
+    for (int i : {1, 3, 5, 6, 7, 8}) sl[ring_index(i) * WP + tt + 1] = v[i];
+    if (t == 0) {
+      const T* h = halo + hk * 3;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) sl[ring_index(kLeftPlanes[q]) * WP] = h[q];
+    }
+    if (t == wv - 1) {
+      const T* h = halo + (kHaloRows + hk) * 3;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) sl[ring_index(kRightPlanes[q]) * WP + wv + 1] = h[q];
+    }
     __syncthreads();
-    if (k >= 2 && active && !(prev_link & kLinkSolid)) {
-      // step 2 at row q = r - 1 (an output row of the tile)
-      const T* s0 = ring + (k & (kRingSlots - 1)) * kRingPlanes * W;        // row q + 1
-      const T* s1 = ring + ((k - 1) & (kRingSlots - 1)) * kRingPlanes * W;  // row q
-      const T* s2 = ring + ((k - 2) & (kRingSlots - 1)) * kRingPlanes * W;  // row q - 1
-      const T* hl = halo + k * 3;                // left halo, row q + 1 (k), q (k - 1), q - 1
-      const T* hr = halo + ((H + 2) + k) * 3;
-      const bool first = t == 0, last = t == wv - 1;
-      const int tl = first ? 0 : t - 1, tr = last ? t : t + 1;  // in-range LDS columns
+    if (two) {
+      // step 2 at row q = r - 1 (an output row of the tile); ring column tt is x - 1, tt + 2
+      // is x + 1
+      const T* s0 = sl;                                                  // row q + 1
+      const T* s1 = ring + ((slot + 3) & (kRingSlots - 1)) * kRingPlanes * WP;  // row q
+      const T* s2 = ring + ((slot + 2) & (kRingSlots - 1)) * kRingPlanes * WP;  // row q - 1
       T w[kQ];
       w[0] = prev[0];
-      w[1] = first ? hl[-3 + 0] : s1[ring_index(1) * W + tl];
+      w[1] = s1[ring_index(1) * WP + tt];
       w[2] = prev2;
-      w[3] = last ? hr[-3 + 0] : s1[ring_index(3) * W + tr];
+      w[3] = s1[ring_index(3) * WP + tt + 2];
       w[4] = v[4];
-      w[5] = first ? hl[-6 + 1] : s2[ring_index(5) * W + tl];
-      w[6] = last ? hr[-6 + 1] : s2[ring_index(6) * W + tr];
-      w[7] = last ? hr[2] : s0[ring_index(7) * W + tr];
-      w[8] = first ? hl[2] : s0[ring_index(8) * W + tl];
+      w[5] = s2[ring_index(5) * WP + tt];
+      w[6] = s2[ring_index(6) * WP + tt + 2];
+      w[7] = s0[ring_index(7) * WP + tt + 2];
+      w[8] = s0[ring_index(8) * WP + tt];
+      const uint32_t bb = prev_link & kLinkBounce;
+      if (__builtin_amdgcn_ballot_w64(bb != 0)) {  // wave-uniform: only rows next to a wall
 #pragma unroll
-      for (int i = 1; i < kQ; ++i)
-        if (prev_link & (1u << i)) w[i] = prev[kOpp[i]];
+        for (int i = 1; i < kQ; ++i)
+          if (bb & (1u << i)) w[i] = prev[kOpp[i]];
+      }
       collide(w, p.fx, p.fy, p.omega, p.fp);
-      const int q = y0 - 2 + k;
-      const unsigned vo = static_cast<unsigned>(c) * B, so = static_cast<unsigned>(q * nx) * B;
+      const unsigned so = static_cast<unsigned>(
+          __builtin_amdgcn_readfirstlane(static_cast<unsigned>((y0 - 2 + k) * nx) * B));
+      const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
 #pragma unroll
-      for (int i = 0; i < kQ; ++i) gn.template store<T, kStoreAux>(i, vo, so, w[i]);
+      for (int i = 0; i < kQ; ++i) gn.template store<T, kStoreAux>(i, vs, so, w[i]);
     }
     prev2 = prev[2];
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) prev[i] = v[i];
+    for (int i = 0; i < kQ; ++i) {
+      prev[i] = v[i];
+      cur[i] = nxt[i];
+    }
     prev_link = link;
+  };
+
+  row(0, 0, 0, false);
+  row(1, 1, 1, false);
+  // Four rows per trip: constant ring slots (LDS offsets), and the copies between the rows'
+  // register sets disappear (unrolling by 1 or 2, or capping the registers at 80 for a fourth
+  // fp32 workgroup per CU, measured 3-5 % slower). Halo chunks: rows 0 .. 33, then 32 rows
+  // from 34 + j * 32 (trip starts: 2 mod 4); a chunk's entries are dead once its last row has
+  // passed its barrier.
+  int k = 2, hk = 2, hcap = kHaloRows;
+  for (; k + 4 <= nr; k += 4, hk += 4) {  // ring slots 2, 3, 0, 1
+    if (hk == hcap) {
+      halo_chunk(k, min(kHaloChunk, nr - k));
+      hk = 0;
+      hcap = kHaloChunk;
+    }
+    row(k, hk, 2, true);
+    row(k + 1, hk + 1, 3, true);
+    row(k + 2, hk + 2, 0, true);
+    row(k + 3, hk + 3, 1, true);
   }
+  for (; k < nr; ++k, ++hk) {
+    if (hk == hcap) {
+      halo_chunk(k, min(kHaloChunk, nr - k));
+      hk = 0;
+      hcap = kHaloChunk;
+    }
+    row(k, hk, k & (kRingSlots - 1), true);
+  }
+}
+
+template <class T, class PS>
+void launch_planes(const StepParams<T>& p, unsigned grid, int block_x, size_t lds, hipStream_t stream) {
+  if (p.store_nt)
+    step2_kernel<T, PS, kAuxNt><<<grid, block_x, lds, stream>>>(p);
+  else
+    step2_kernel<T, PS, kAuxNone><<<grid, block_x, lds, stream>>>(p);
 }
 
 }  // namespace
 
-size_t step2_lds_bytes(int block_x, int rows, size_t elem) {
-  return (static_cast<size_t>(kRingSlots) * kRingPlanes * block_x + 2 * static_cast<size_t>(rows + 2) * 3) * elem;
+size_t step2_lds_bytes(int block_x, size_t elem) {
+  return (static_cast<size_t>(kRingSlots) * kRingPlanes * (block_x + 2) + 2 * kHaloRows * 3) * elem;
 }
 
 template <class T>
 hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream) {
-  // periodic, uniform force, one buffer resource, whole domain, tile rows >= 1
+  // periodic, uniform force, whole domain, tile rows >= 1, a plane addressable with 32 bits
+  // and holding the discard slots
   if (!p.wrap_y || p.force || p.row0 != 0 || p.nrows != p.ny || p.rows_per_tile < 1 ||
       block_x < 64 || block_x > 512 || block_x % 64 ||
       p.tiles_x != (p.nx + block_x - 1) / block_x ||
-      static_cast<uint64_t>(kQ) * p.N * sizeof(T) >= (uint64_t{1} << 32))
+      static_cast<uint64_t>(p.N) * sizeof(T) >= (uint64_t{1} << 32) ||
+      p.N < static_cast<size_t>(p.nx) * p.ny + kPlaneTail)
     return hipErrorInvalidValue;
-  const size_t lds = step2_lds_bytes(block_x, p.rows_per_tile, sizeof(T));
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const unsigned tiles_y = static_cast<unsigned>((p.ny + p.rows_per_tile - 1) / p.rows_per_tile);
+  if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
   const unsigned grid = static_cast<unsigned>(p.tiles_x) * tiles_y;
-  if (p.store_nt)
-    step2_kernel<T, kAuxNt><<<grid, block_x, lds, stream>>>(p);
+  const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
+  if (one)
+    launch_planes<T, PlanesOne>(p, grid, block_x, lds, stream);
   else
-    step2_kernel<T, kAuxNone><<<grid, block_x, lds, stream>>>(p);
+    launch_planes<T, Planes>(p, grid, block_x, lds, stream);
   return hipGetLastError();
 }
 template hipError_t launch_step2<float>(const StepParams<float>&, int, hipStream_t);
 template hipError_t launch_step2<double>(const StepParams<double>&, int, hipStream_t);
+
+template <class T>
+int step2_tile_rows(const StepParams<T>& p, int block_x) {
+  // Workgroups of this instance resident at once on the current device.
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
+  const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
+  int per_cu = 0, dev = 0, cus = 0;
+  const hipError_t e =
+      one ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, PlanesOne, kAuxNt>,
+                                                         block_x, lds)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, Planes, kAuxNt>,
+                                                         block_x, lds);
+  if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      per_cu < 1 || cus < 1) {
+    (void)hipGetLastError();
+    return kHaloChunk;
+  }
+  // The tallest candidate whose tiles fill the resident slots evenly (the last wave of
+  // workgroups at least 97 % full), else the best-filling one. 8192^2 fp32, 512 columns, two
+  // workgroups per CU: 64 rows (2048 tiles, four full waves) 146.3k MLUPS; 72 rows (1824 tiles,
+  // last wave 56 % full) 125.8k, 56 rows 128.3k; 32 rows (eight waves, more recomputed rows)
+  // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k.
+  const long slots = static_cast<long>(per_cu) * cus;
+  const long tiles_x = (p.nx + block_x - 1) / block_x;
+  int best = 16;
+  double best_fill = -1.0;
+  for (const int rows : {64, 48, 32, 24, 16}) {
+    const long tiles = tiles_x * ((p.ny + rows - 1) / rows);
+    const double fill = static_cast<double>(tiles) / (((tiles + slots - 1) / slots) * slots);
+    if (fill >= 0.97) return rows;
+    if (fill > best_fill) {
+      best_fill = fill;
+      best = rows;
+    }
+  }
+  return best;
+}
+template int step2_tile_rows<float>(const StepParams<float>&, int);
+template int step2_tile_rows<double>(const StepParams<double>&, int);
 
 }  // namespace lbmhip

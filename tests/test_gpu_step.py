@@ -281,7 +281,12 @@ def test_tile_orders_give_identical_bits(lbm, oracle_mod, prec, order):
 @pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("grid,tile", [((53, 31), (512, 64)), ((53, 31), (64, 1)),
                                        ((200, 70), (64, 7)), ((640, 416), (512, 64)),
-                                       ((640, 416), (128, 3)), ((1000, 300), (256, 16))])
+                                       ((640, 416), (128, 3)), ((1000, 300), (256, 16)),
+                                       # tiles taller than one 32-row halo chunk: several chunks,
+                                       # a partial last one, rows past the four-row trips
+                                       ((640, 416), (512, 150)), ((333, 700), (128, 333)),
+                                       # the automatic height
+                                       ((300, 257), (256, -1))])
 def test_two_step_launches_match_oracle(lbm, oracle_mod, prec, grid, tile):
     """Temporal blocking: run(n) advances pairs of steps in one launch (tiles of block_x
     columns by rows rows, halo columns and rows recomputed), odd n ends with a single step.
@@ -300,7 +305,12 @@ def test_two_step_launches_match_oracle(lbm, oracle_mod, prec, grid, tile):
         tile = (min(tile[0], 256), tile[1])
     s = lbm.Solver(nx, ny, 0.73, 2e-5, -7e-6, solid, prec)
     s.set_steps_per_launch(2, *tile)
-    assert s.steps_per_launch()["in_use"]
+    info = s.steps_per_launch()
+    assert info["in_use"]
+    if tile[1] > 0:
+        assert info["rows"] == tile[1]
+    else:
+        assert info["rows"] >= 16
     o = oracle_mod.Oracle(nx, ny, 0.73, 2e-5, -7e-6, solid, prec, "v0")
     for x in (s, o):
         x.set_populations(S)
@@ -310,6 +320,32 @@ def test_two_step_launches_match_oracle(lbm, oracle_mod, prec, grid, tile):
         o.run(n)
         done += n
         assert_bitwise(s.populations(), o.populations(), f"{grid} tile {tile} after {done} steps")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_two_step_raw_state_matches_single_steps(lbm, prec):
+    """The raw device state (pull form, solid cells included) after two-step launches equals
+    the single-step path's bit for bit: solid cells keep the values the upload gave them (their
+    results go to the discard slots past each plane), fluid cells match."""
+    nx, ny = 300, 90
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[30:45, 100:130] = 1
+    solid = solid.reshape(-1)
+    rng = np.random.default_rng(11)
+    dt = np.float32 if prec == "float" else np.float64
+    R = (rng.standard_normal(nx * ny * 9) * 1e-2).astype(dt)  # solid cells hold values too
+    out = []
+    for spl in (2, 1):
+        s = lbm.Solver(nx, ny, 0.7, 1e-5, 3e-6, solid, prec)
+        s.set_steps_per_launch(spl, 256 if prec == "double" else 512, 40)
+        assert s.steps_per_launch()["in_use"] == (spl == 2)
+        s.set_state(R)
+        s.run(6)
+        out.append(s.get_state())
+        s.close()
+    assert_bitwise(out[0], out[1], "two-step vs single-step raw state")
+    cells = np.asarray(R).reshape(9, ny * nx)[:, solid.astype(bool)]
+    assert_bitwise(out[0].reshape(9, ny * nx)[:, solid.astype(bool)], cells, "solid cells kept")
 
 
 def test_two_step_path_is_the_default_where_it_applies(lbm):
