@@ -141,8 +141,11 @@ int lbm_dims(const lbm_solver* s, int* nx, int* ny, int* precision);
 /* FP16 storage scale (power of two): stored half = g * scale. Default 2^12. */
 int lbm_set_fp16_scale(lbm_solver* s, double scale);
 
-/* Autotuner: times `steps` steps for each candidate (block_x, rows_per_thread) and keeps the
- * fastest on the handle. The state is saved and restored around the trials. */
+/* Autotuner: times `steps` steps for each candidate (block_x, rows_per_thread) and code option
+ * of the one-step kernel and keeps the fastest on the handle; where run() takes the two-step
+ * kernel it then times its tiles (256 or 512 columns; automatic, 32 or 64 rows; tile orders
+ * 0..2) and keeps the fastest, and best_mlups is the rate of that path. The state is saved and
+ * restored around the trials. */
 int lbm_autotune(lbm_solver* s, int steps, int* best_block_x, int* best_rows, double* best_mlups);
 
 /* No reference counterpart (the reference samples a velocity probe, driver.hpp:186-251):

@@ -358,3 +358,32 @@ def test_two_step_path_is_the_default_where_it_applies(lbm):
     s3 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "double")
     s3.set_force_field(np.zeros(2 * 256 * 64))
     assert not s3.steps_per_launch()["in_use"]  # per-cell force field: single steps
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_autotune_restores_state_and_keeps_results(lbm, prec):
+    """lbm_autotune times every one-step shape and option and, where run() takes it, every
+    two-step tile; the state is restored and the tuned solver computes the untuned one's bits."""
+    nx, ny = 640, 200
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[80:100, 300:330] = 1
+    solid = solid.reshape(-1)
+    rng = np.random.default_rng(5)
+    S = (rng.standard_normal(nx * ny * 9) * 1e-3).astype(
+        {"float": np.float32, "double": np.float64, "fp16": np.float32}[prec])
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    out = []
+    for tune in (True, False):
+        s = lbm.Solver(nx, ny, 0.7, 1e-5, 0.0, solid, prec)
+        s.set_populations(S)
+        if tune:
+            before = s.populations()
+            bx, r, mlups = s.autotune(steps=2)
+            assert mlups > 0 and bx % 64 == 0 and r >= 1
+            assert_bitwise(s.populations(), before, "state restored after autotune")
+            info = s.steps_per_launch()
+            assert info["in_use"] == (prec != "fp16")
+        s.run(7)
+        out.append(s.populations())
+        s.close()
+    assert_bitwise(out[0], out[1], "tuned vs untuned")
