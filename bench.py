@@ -156,17 +156,22 @@ def form_batched(lbm, args):
     b = lbm.Batch(doms, "float")
     tx, tr, wg, tuned = b.autotune(steps=3)
     order, nt = b.order()
+    two = b.steps_per_launch()
     b.set_graph_steps(100)
     b.run(100)
     t = b.timed_run(1000)
     cells = b.total_cells()
     mlups = cells * 1000 / t / 1e6
     b.close()
-    log(f"batched: {mlups:.0f} MLUPS over {cells} cells (order {order}, tile {tx}x{tr}, {wg} WGs)")
+    per_launch = two["steps_per_launch"]
+    log(f"batched: {mlups:.0f} MLUPS over {cells} cells ({per_launch} steps per launch, two-step "
+        f"tile {two['tile_x']}x{two['tile_rows']}; one-step order {order}, tile {tx}x{tr}, {wg} WGs)")
     return {"domains": 256, "seed": 0, "cells": cells, "mlups": round(mlups, 1),
-            "timed_steps": 1000, "graph_steps": 100, "order": order, "streaming_stores": nt,
-            "tile_x": tx, "tile_rows": tr, "workgroups": wg,
-            "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / 1e9, 1)}
+            "timed_steps": 1000, "graph_steps": 100, "steps_per_launch": per_launch,
+            "two_step_tile": f"{two['tile_x']}x{two['tile_rows']}",
+            "one_step": {"order": order, "streaming_stores": nt, "tile_x": tx, "tile_rows": tr,
+                         "workgroups": wg},
+            "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / per_launch / 1e9, 1)}
 
 
 def form_strip(lbm, args):

@@ -13,7 +13,7 @@ struct BatchDomain {
   size_t offset;       // element offset of plane 0 in the population buffers
   size_t cell_offset;  // offset in the link-code array
   size_t seg_offset;   // offset in the segment-flag array
-  size_t N;            // cells (plane stride)
+  size_t N;            // plane stride (cells rounded up to 64, plus kPlaneTail)
   int nx, ny, nseg, pad;
   T omega, fp, fx, fy;
 };
@@ -53,5 +53,11 @@ struct BatchParams {
 
 template <class T>
 hipError_t launch_batch(const BatchParams<T>& p, int block_x, int workgroups, hipStream_t stream);
+
+// Two steps per launch over the batch (step2.hip): p.tiles is the two-step work-list (tiles of
+// block_x columns by p.tile_rows rows), one workgroup per tile; every domain's plane stride
+// holds kPlaneTail values past its cells. Bit-identical to two launch_batch steps.
+template <class T>
+hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, hipStream_t stream);
 
 }  // namespace lbmhip

@@ -25,6 +25,8 @@ struct DomainHost {
   size_t offset = 0, cell_offset = 0, seg_offset = 0;
   int nseg = 0;
   size_t N() const { return static_cast<size_t>(nx) * ny; }
+  // plane stride: the cells rounded up to 64, plus the two-step kernel's discard slots
+  size_t S() const { return (N() + 63) / 64 * 64 + kPlaneTail; }
 };
 
 constexpr size_t kAlignElems = 64;  // plane-0 offsets aligned to 64 elements
@@ -46,6 +48,12 @@ struct lbm_batch {
   size_t segs_cap = 0;
   int ntiles = 0;             // entries of the list the current order walks
   int tile_x = 256, tile_rows = 4, workgroups = 0;
+  // two steps per launch: its own tile list (tile2_x columns by tile2_rows rows)
+  int steps_per_launch = 2;
+  int tile2_x = 256, tile2_rows = 32;
+  BatchTile* dtiles2 = nullptr;
+  size_t tiles2_cap = 0;
+  int ntiles2 = 0;
   int order = 0, store_nt = 1;  // tile order (see batch.hip) and non-temporal stores
   bool one_rsrc = true;         // every domain's nine planes span < 4 GiB
   int graph_steps = 100;
@@ -70,7 +78,8 @@ struct lbm_batch {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     for (void* p : {g, gn, static_cast<void*>(links), static_cast<void*>(segflags), ddesc,
-                    static_cast<void*>(dtiles), static_cast<void*>(dsegs), staging})
+                    static_cast<void*>(dtiles), static_cast<void*>(dsegs),
+                    static_cast<void*>(dtiles2), staging})
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -86,7 +95,7 @@ struct lbm_batch {
       b.offset = h.offset;
       b.cell_offset = h.cell_offset;
       b.seg_offset = h.seg_offset;
-      b.N = h.N();
+      b.N = h.S();
       b.nx = h.nx;
       b.ny = h.ny;
       b.nseg = h.nseg;
@@ -143,6 +152,69 @@ struct lbm_batch {
     return LBM_OK;
   }
 
+  // Two-step work-list: tiles of tile2_x columns by tile2_rows rows, domain by domain.
+  int build_tiles2() {
+    std::vector<BatchTile> t;
+    for (size_t k = 0; k < doms.size(); ++k) {
+      const int tx = (doms[k].nx + tile2_x - 1) / tile2_x;
+      const int ty = (doms[k].ny + tile2_rows - 1) / tile2_rows;
+      for (int y = 0; y < ty; ++y)
+        for (int x = 0; x < tx; ++x) t.push_back(BatchTile{static_cast<int>(k), x, y, 0});
+    }
+    ntiles2 = static_cast<int>(t.size());
+    if (t.size() > tiles2_cap) {
+      if (dtiles2) LBM_HIP_TRY(hipFree(dtiles2));
+      dtiles2 = nullptr;
+      if (hipMalloc(&dtiles2, t.size() * sizeof(BatchTile)) != hipSuccess)
+        return fail(LBM_ENOMEM, "device allocation of the two-step tile list failed");
+      tiles2_cap = t.size();
+    }
+    LBM_HIP_TRY(hipMemcpy(dtiles2, t.data(), t.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
+    drop_graphs();
+    return LBM_OK;
+  }
+
+  bool two_step() const {
+    return steps_per_launch == 2 && step2_lds_bytes(tile2_x, rs()) <= 64 * 1024;
+  }
+
+  template <class T>
+  BatchParams<T> params2(bool flip) const {
+    BatchParams<T> p = params<T>(flip);
+    p.tiles = dtiles2;
+    p.ntiles = ntiles2;
+    p.tile_rows = tile2_rows;
+    return p;
+  }
+
+  int launch2(bool flip) {
+    const hipError_t e = prec == LBM_DOUBLE ? launch_batch_step2(params2<double>(flip), tile2_x, stream)
+                                            : launch_batch_step2(params2<float>(flip), tile2_x, stream);
+    if (e != hipSuccess) return fail(LBM_EHIP, std::string("two-step batch launch: ") + hipGetErrorString(e));
+    return LBM_OK;
+  }
+
+  // Launches advancing `steps` steps from buffer parity par (updated): pairs in two-step
+  // launches when on, one-step launches otherwise and for an odd remainder.
+  int issue(int steps, int& par) {
+    if (two_step())
+      for (; steps >= 2; steps -= 2) {
+        LBM_TRY(launch2(par == 1));
+        par ^= 1;
+      }
+    for (; steps > 0; --steps) {
+      LBM_TRY(launch(par == 1));
+      par ^= 1;
+    }
+    return LBM_OK;
+  }
+
+  // Parity change of a graph of graph_steps steps (one per launch).
+  int graph_flip() const {
+    const int launches = two_step() ? graph_steps / 2 + (graph_steps & 1) : graph_steps;
+    return launches & 1;
+  }
+
   template <class T>
   BatchParams<T> params(bool flip) const {
     BatchParams<T> p{};
@@ -185,11 +257,7 @@ struct lbm_batch {
     hipGraph_t graph = nullptr;
     LBM_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int par = p;
-    int rc = LBM_OK;
-    for (int k = 0; k < graph_steps && rc == LBM_OK; ++k) {
-      rc = launch(par == 1);
-      par ^= 1;
-    }
+    const int rc = issue(graph_steps, par);
     const hipError_t e = hipStreamEndCapture(stream, &graph);
     if (rc != LBM_OK) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -206,11 +274,10 @@ struct lbm_batch {
     while (steps >= graph_steps && graph_steps > 1) {
       if (!exec[parity]) LBM_TRY(build_graph(parity));
       LBM_HIP_TRY(hipGraphLaunch(exec[parity], stream));
-      parity ^= graph_steps & 1;
+      parity ^= graph_flip();
       steps -= graph_steps;
     }
-    for (int k = 0; k < steps; ++k) LBM_TRY(step_direct());
-    return LBM_OK;
+    return issue(steps, parity);
   }
 
   void* current() const { return parity == 0 ? g : gn; }
@@ -251,10 +318,12 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
     h.offset = b->total_elems;
     h.cell_offset = b->total_cells;
     h.seg_offset = b->total_segs;
-    b->total_elems += (kQ * h.N() + kAlignElems - 1) / kAlignElems * kAlignElems;
+    b->total_elems += (kQ * h.S() + kAlignElems - 1) / kAlignElems * kAlignElems;
     b->total_cells += h.N();
     b->total_segs += static_cast<size_t>(h.ny) * h.nseg;
-    if (kQ * h.N() * b->rs() >= (size_t(1) << 32)) b->one_rsrc = false;
+    if (kQ * h.S() * b->rs() >= (size_t(1) << 32)) b->one_rsrc = false;
+    if (h.S() * b->rs() >= (size_t(1) << 32))
+      return bail(fail(LBM_EINVAL, "domain " + std::to_string(k) + ": a plane must stay below 4 GiB"));
   }
   int rc = parse_block_env(&b->tile_x, &b->tile_rows);
   if (rc != LBM_OK) return bail(rc);
@@ -265,10 +334,12 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   b->workgroups = std::max(1, cus) * 8;
+  b->tile2_x = precision == LBM_DOUBLE ? 128 : 256;
   const size_t bytes = b->total_elems * b->rs();
+  // segment flags in whole dwords: the two-step kernel reads a flag with a scalar dword load
   if (hipMalloc(&b->g, bytes) != hipSuccess || hipMalloc(&b->gn, bytes) != hipSuccess ||
       hipMalloc(&b->links, b->total_cells * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&b->segflags, b->total_segs) != hipSuccess ||
+      hipMalloc(&b->segflags, (b->total_segs + 3) & ~size_t(3)) != hipSuccess ||
       hipMalloc(&b->ddesc, n * std::max(sizeof(BatchDomain<double>), sizeof(BatchDomain<float>))) != hipSuccess)
     return bail(fail(LBM_ENOMEM, "device allocation of the batch failed"));
   // link codes, flags and the initial state of every domain
@@ -279,17 +350,23 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
     const Links L = build_links(h.nx, h.ny, h.solid.data(), false, true);
     std::copy(L.code.begin(), L.code.end(), codes.begin() + h.cell_offset);
     std::copy(L.seg.begin(), L.seg.end(), segs.begin() + h.seg_offset);
+    // the preimage is computed with planes nx * ny apart, then laid out at the plane stride
+    auto place = [&](auto* R, auto* dst) {
+      for (int i = 0; i < kQ; ++i) std::copy(R + i * h.N(), R + (i + 1) * h.N(), dst + i * h.S());
+    };
     if (precision == LBM_DOUBLE) {
       double eq[kQ];
       equilibrium_state<double>(h.rho0, h.ux0, h.uy0, eq);
-      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq,
-                       reinterpret_cast<double*>(state.data()) + h.offset);
+      std::vector<double> R(kQ * h.N());
+      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq, R.data());
+      place(R.data(), reinterpret_cast<double*>(state.data()) + h.offset);
     } else {
       float eq[kQ];
       equilibrium_state<float>(static_cast<float>(h.rho0), static_cast<float>(h.ux0),
                                static_cast<float>(h.uy0), eq);
-      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq,
-                       reinterpret_cast<float*>(state.data()) + h.offset);
+      std::vector<float> R(kQ * h.N());
+      preimage_uniform(h.nx, h.ny, h.solid.data(), false, eq, R.data());
+      place(R.data(), reinterpret_cast<float*>(state.data()) + h.offset);
     }
   }
   if (hipMemcpy(b->links, codes.data(), codes.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
@@ -299,6 +376,7 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
     return bail(fail(LBM_EHIP, "upload of the batch failed"));
   rc = precision == LBM_DOUBLE ? b->upload_descriptors<double>() : b->upload_descriptors<float>();
   if (rc == LBM_OK) rc = b->build_tiles();
+  if (rc == LBM_OK) rc = b->build_tiles2();
   if (rc != LBM_OK) return bail(rc);
   *out = b;
   return LBM_OK;
@@ -346,6 +424,32 @@ int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores) {
   return LBM_OK;
 }
 
+int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  if (steps_per_launch != 1 && steps_per_launch != 2)
+    return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
+  if (tile_x != 0 && (tile_x < 64 || tile_x > 512 || tile_x % 64))
+    return fail(LBM_EINVAL, "tile_x must be 0 or a multiple of 64 in 64..512");
+  if (tile_x != 0 && step2_lds_bytes(tile_x, b->rs()) > 64 * 1024)
+    return fail(LBM_EINVAL, "two-step tile too wide for this precision (LDS)");
+  if (tile_rows < 0) return fail(LBM_EINVAL, "tile_rows must be >= 0");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->steps_per_launch = steps_per_launch;
+  if (tile_x) b->tile2_x = tile_x;
+  if (tile_rows) b->tile2_rows = tile_rows;
+  b->drop_graphs();
+  return b->build_tiles2();
+}
+
+int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, int* tile_x,
+                                   int* tile_rows) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  if (steps_per_launch) *steps_per_launch = b->two_step() ? 2 : 1;
+  if (tile_x) *tile_x = b->tile2_x;
+  if (tile_rows) *tile_rows = b->tile2_rows;
+  return LBM_OK;
+}
+
 int lbm_batch_run(lbm_batch* b, int steps) {
   if (!b || steps < 0) return fail(LBM_EINVAL, "null batch handle or negative steps");
   LBM_TRY(b->run_async(steps));
@@ -358,7 +462,7 @@ int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds) {
   // Graphs are built before the timed region (instantiation is not part of a step).
   if (steps >= b->graph_steps && b->graph_steps > 1) {
     if (!b->exec[b->parity]) LBM_TRY(b->build_graph(b->parity));
-    if ((b->graph_steps & 1) && !b->exec[b->parity ^ 1]) LBM_TRY(b->build_graph(b->parity ^ 1));
+    if (b->graph_flip() && !b->exec[b->parity ^ 1]) LBM_TRY(b->build_graph(b->parity ^ 1));
   }
   LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
   LBM_TRY(b->run_async(steps));
@@ -387,7 +491,7 @@ int lbm_batch_populations(lbm_batch* b, int domain, void* out) {
     GatherParams<double> p{};
     p.g = static_cast<const double*>(b->current()) + h.offset;
     p.links = b->links + h.cell_offset;
-    p.N = h.N();
+    p.N = h.S();
     p.nx = h.nx;
     p.ny = h.ny;
     p.fx = h.fx;
@@ -397,7 +501,7 @@ int lbm_batch_populations(lbm_batch* b, int domain, void* out) {
     GatherParams<float> p{};
     p.g = static_cast<const float*>(b->current()) + h.offset;
     p.links = b->links + h.cell_offset;
-    p.N = h.N();
+    p.N = h.S();
     p.nx = h.nx;
     p.ny = h.ny;
     p.fx = static_cast<float>(h.fx);
@@ -468,6 +572,38 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
   b->order = border;
   b->store_nt = bnt;
   LBM_TRY(b->build_tiles());
+  // The two-step tiles, where run() takes them; best_mlups is then the rate of that path.
+  if (b->steps_per_launch == 2) {
+    double best2 = -1.0;
+    int b2x = b->tile2_x, b2r = b->tile2_rows;
+    for (int tx : {64, 128, 256, 512})
+      for (int r : {16, 32, 64}) {
+        if (step2_lds_bytes(tx, b->rs()) > 64 * 1024) continue;
+        b->tile2_x = tx;
+        b->tile2_rows = r;
+        LBM_TRY(b->build_tiles2());
+        int par = b->parity;
+        LBM_TRY(b->issue(2, par));  // warm-up
+        LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        LBM_TRY(b->issue(2 * ((steps + 1) / 2), par));
+        LBM_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+        LBM_HIP_TRY(hipEventSynchronize(b->ev1));
+        b->parity = par;
+        float ms = 0.0f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        const double mlups =
+            static_cast<double>(b->total_cells) * 2 * ((steps + 1) / 2) / (ms * 1e-3) / 1e6;
+        if (mlups > best2) {
+          best2 = mlups;
+          b2x = tx;
+          b2r = r;
+        }
+      }
+    b->tile2_x = b2x;
+    b->tile2_rows = b2r;
+    LBM_TRY(b->build_tiles2());
+    if (best2 > 0.0) best = best2;
+  }
   b->parity = keep_parity;
   LBM_HIP_TRY(hipMemcpy(b->current(), backup, bytes, hipMemcpyDeviceToDevice));
   LBM_HIP_TRY(hipFree(backup));

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "step_body.hpp"
 
@@ -63,22 +64,49 @@ __device__ __forceinline__ Cols cols_of(int c, int nx, unsigned B) {
               static_cast<unsigned>(c == nx - 1 ? 0 : c + 1) * B};
 }
 
-// Segment flag at a wave-uniform byte index, by a scalar load of its aligned dword (the flag
-// array is allocated in whole dwords): the branch on it is a scalar branch, and the wait for
-// it does not wait for the wave's outstanding vector stores as a vector load's wait would.
+// Segment flag at a wave-uniform byte index, by a scalar load of the aligned dword holding it
+// (flag arrays are allocated in whole dwords): the branch on it is a scalar branch, and the
+// wait for it does not wait for the wave's outstanding vector stores as a vector load's wait
+// would.
 __device__ __forceinline__ uint32_t uniform_flag(const uint8_t* flags, size_t index) {
   using ConstU32 = const __attribute__((address_space(4))) uint32_t*;
   const size_t at = static_cast<size_t>(__builtin_amdgcn_readfirstlane(static_cast<unsigned>(index)));
-  const uint32_t w = *(ConstU32)(flags + (at & ~size_t(3)));
-  return (w >> ((at & 3u) * 8u)) & 0xffu;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(flags) + at;
+  const uint32_t w = *(ConstU32)(a & ~uintptr_t(3));
+  return (w >> ((a & 3u) * 8u)) & 0xffu;
 }
+
+// A descriptor read with scalar loads: the caller's index is wave-uniform and the table is not
+// written while kernels run.
+template <class S>
+__device__ __forceinline__ S load_uniform(const S* ptr) {
+  static_assert(sizeof(S) % 4 == 0, "descriptor of whole dwords");
+  using ConstU32 = const __attribute__((address_space(4))) uint32_t*;
+  const ConstU32 src = (ConstU32)(ptr);
+  uint32_t w[sizeof(S) / 4];
+#pragma unroll
+  for (size_t i = 0; i < sizeof(S) / 4; ++i) w[i] = src[i];
+  return __builtin_bit_cast(S, w);
+}
+
+// One periodic domain with a uniform force: the plain solver's grid or one batched domain.
+template <class T>
+struct Step2View {
+  const T* g;
+  T* gn;
+  const uint16_t* links;
+  const uint8_t* segflags;
+  size_t N;  // plane stride, >= nx * ny + kPlaneTail
+  int nx, ny, nseg;
+  T omega, fp, fx, fy;
+};
 
 // Step 1 at a halo-column cell (c, r) (both wrapped) whose row differs between the lanes of a
 // wave: the row offsets go into the per-lane offset (a per-lane scalar offset would make the
 // compiler loop over its values). The pull with its bounce-back corrections and the collision,
 // exactly as update_column.
 template <class T, class PS>
-__device__ __forceinline__ void halo_step(const StepParams<T>& p, const PS& g, int c, int r,
+__device__ __forceinline__ void halo_step(const Step2View<T>& p, const PS& g, int c, int r,
                                           T (&v)[kQ]) {
   constexpr unsigned B = sizeof(T);
   const Rows rows = rows_of(r, p.nx, p.ny, B);
@@ -102,20 +130,19 @@ __device__ __forceinline__ void halo_step(const StepParams<T>& p, const PS& g, i
 constexpr int kHaloChunk = 32;
 constexpr int kHaloRows = kHaloChunk + 2;
 
+// The walk of one tile: columns x0 .. x0 + blockDim.x - 1 (those inside the grid), output rows
+// y0 .. y0 + H - 1 (those inside the grid). Ends with every LDS read done before the next walk's
+// first LDS write crosses a barrier (its halo chunk ends with one).
 template <class T, class PS, int kStoreAux>
-__global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, const int y0,
+                                           const int H, unsigned char* smem) {
   const int W = static_cast<int>(blockDim.x), WP = W + 2;
-  const int H = p.rows_per_tile;
   T* ring = reinterpret_cast<T*>(smem);                  // [kRingSlots][kRingPlanes][W + 2]
   T* halo = ring + kRingSlots * kRingPlanes * WP;        // [2][kHaloRows][3]
   const int t = static_cast<int>(threadIdx.x);
   const int nx = p.nx, ny = p.ny;
-  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
-  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
-  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
-  const int x0 = tx * W, wv = min(W, nx - x0);  // valid columns of this tile
-  const int y0 = ty * H, y1 = min(y0 + H, ny);
+  const int wv = min(W, nx - x0);  // valid columns of this tile
+  const int y1 = min(y0 + H, ny);
   const int nr = y1 - y0 + 2;                   // step-1 rows y0 - 1 .. y1
   PS g, gn;
   g.init(p.g, p.N);
@@ -267,6 +294,32 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
   }
 }
 
+template <class T, class PS, int kStoreAux>
+__global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
+  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
+  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
+  const Step2View<T> d{p.g,  p.gn,   p.links, p.segflags, p.N,  p.nx,
+                       p.ny, p.nseg, p.omega, p.fp,       p.fx, p.fy};
+  step2_walk<T, PS, kStoreAux>(d, tx * static_cast<int>(blockDim.x), ty * p.rows_per_tile,
+                               p.rows_per_tile, smem);
+}
+
+// Batched domains: workgroup b walks tile b of the two-step work-list (domain, column tile, row
+// tile of p.tile_rows rows), with that domain's descriptor.
+template <class T, class PS, int kStoreAux>
+__global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BatchTile tile = load_uniform(p.tiles + blockIdx.x);
+  const BatchDomain<T> dm = load_uniform(p.domains + tile.domain);
+  const Step2View<T> d{p.g + dm.offset, p.gn + dm.offset, p.links + dm.cell_offset,
+                       p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg,
+                       dm.omega, dm.fp, dm.fx, dm.fy};
+  step2_walk<T, PS, kStoreAux>(d, tile.tx * static_cast<int>(blockDim.x), tile.ty * p.tile_rows,
+                               p.tile_rows, smem);
+}
+
 template <class T, class PS>
 void launch_planes(const StepParams<T>& p, unsigned grid, int block_x, size_t lds, hipStream_t stream) {
   if (p.store_nt)
@@ -305,6 +358,31 @@ hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream)
 }
 template hipError_t launch_step2<float>(const StepParams<float>&, int, hipStream_t);
 template hipError_t launch_step2<double>(const StepParams<double>&, int, hipStream_t);
+
+template <class T>
+hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, hipStream_t stream) {
+  // every domain periodic with a uniform force and the plane tail (the batch layout has both)
+  if (p.ntiles <= 0) return hipSuccess;
+  if (p.tile_rows < 1 || block_x < 64 || block_x > 512 || block_x % 64)
+    return hipErrorInvalidValue;
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const unsigned grid = static_cast<unsigned>(p.ntiles);
+  if (p.one_rsrc) {
+    if (p.store_nt)
+      batch_step2_kernel<T, PlanesOne, kAuxNt><<<grid, block_x, lds, stream>>>(p);
+    else
+      batch_step2_kernel<T, PlanesOne, kAuxNone><<<grid, block_x, lds, stream>>>(p);
+  } else {
+    if (p.store_nt)
+      batch_step2_kernel<T, Planes, kAuxNt><<<grid, block_x, lds, stream>>>(p);
+    else
+      batch_step2_kernel<T, Planes, kAuxNone><<<grid, block_x, lds, stream>>>(p);
+  }
+  return hipGetLastError();
+}
+template hipError_t launch_batch_step2<float>(const BatchParams<float>&, int, hipStream_t);
+template hipError_t launch_batch_step2<double>(const BatchParams<double>&, int, hipStream_t);
 
 template <class T>
 int step2_tile_rows(const StepParams<T>& p, int block_x) {

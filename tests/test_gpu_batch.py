@@ -93,3 +93,39 @@ def test_batch_rejects_invalid_domains(lbm):
     bad = dict(d, tau=0.5)
     with pytest.raises(ValueError, match="tau"):
         lbm.Batch([d, bad])
+
+
+@pytest.mark.parametrize("prec", ("float", "double"))
+def test_two_step_batch_matches_one_step_batch(lbm, oracle_mod, prec):
+    """Two steps per launch over the batch (one workgroup per tile, per-domain descriptors):
+    tile shapes from narrow to wider than most domains, graphs of both parities, direct pairs and
+    an odd remainder; every domain equals the one-step batch and, for two, the oracle."""
+    doms = lbm.random_domains(seed=21, n=30, nx_range=(20, 700), ny_range=(9, 200),
+                              radius_range=(2, 12))
+    init = (1.0, 0.02, -0.01)
+    out = []
+    for spl in (2, 1):
+        b = lbm.Batch(doms, prec, init=init)
+        b.set_graph_steps(6)
+        if spl == 2:
+            for tile, steps in (((64, 16), 6), ((256, 32), 5), ((512, 64), 7)):
+                if prec == "double":  # its LDS ring holds twice the bytes: at most 256 columns
+                    tile = (min(tile[0], 256), tile[1])
+                b.set_steps_per_launch(2, *tile)
+                assert b.steps_per_launch() == {"steps_per_launch": 2, "tile_x": tile[0],
+                                                "tile_rows": tile[1]}
+                b.run(steps)
+        else:
+            b.set_steps_per_launch(1)
+            assert b.steps_per_launch()["steps_per_launch"] == 1
+            b.run(18)
+        out.append([b.populations(k) for k in range(len(doms))])
+        b.close()
+    for k in range(len(doms)):
+        assert np.count_nonzero(bits(out[0][k]) != bits(out[1][k])) == 0, k
+    for k in (0, 1):
+        d = doms[k]
+        o = oracle_mod.Oracle(d["nx"], d["ny"], d["tau"], d["fx"], d["fy"], d["solid"], prec, "v0")
+        o.init_equilibrium(*init)
+        o.run(18)
+        assert np.count_nonzero(bits(o.populations()) != bits(out[0][k])) == 0, k
