@@ -41,14 +41,25 @@ __host__ __device__ constexpr uint32_t outlet_bb_bit(int i) {
 // Moments with the Guo half-force velocity, then BGK relaxation plus the Guo source, in place.
 // Every product by a lattice constant is written as in the reference (int constant converted
 // to T); the compiler may only apply IEEE-exact folds (1*x, -1*x, x + -y), never 0*x.
+//
+// Two terms of the reference's expressions are left out because they cannot change a bit of
+// any result that is finite (a non-finite state has no bitwise counterpart: NaN encodings
+// differ between processors):
+//  * the products 0 * g_i in the sums over c_x g_i and c_y g_i: each running sum starts at +0,
+//    and a round-to-nearest sum is -0 only when both operands are -0, so the sum is never -0
+//    and adding +-0 to it is the identity;
+//  * 3 cu + 4.5 cu cu of the rest direction: there cu = 0 * ux + 0 * uy = +-0, so 3 cu is +-0,
+//    4.5 cu * cu is +0 and their sum is +0 whatever the signs; the bracket is +0 - 1.5 usq.
+// Both are checked bit for bit against the reference's full expressions on random and
+// signed-zero / subnormal inputs (tests/cpp/test_collide.cpp).
 template <class T>
 __host__ __device__ __forceinline__ void collide(T (&g)[kQ], T fx, T fy, T omega, T fp) {
   T m0 = T(0), mx = T(0), my = T(0);
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
     m0 += g[i];
-    mx += T(kCx[i]) * g[i];
-    my += T(kCy[i]) * g[i];
+    if (kCx[i] != 0) mx += T(kCx[i]) * g[i];
+    if (kCy[i] != 0) my += T(kCy[i]) * g[i];
   }
   const T drho = m0;
   const T rho = T(1) + drho;
@@ -59,7 +70,8 @@ __host__ __device__ __forceinline__ void collide(T (&g)[kQ], T fx, T fy, T omega
   for (int i = 0; i < kQ; ++i) {
     const T cxi = T(kCx[i]), cyi = T(kCy[i]), wi = weight<T>(i);
     const T cu = cxi * ux + cyi * uy;
-    const T geq = wi * (drho + rho * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq));
+    const T bracket = i == 0 ? T(0) - T(1.5) * usq : T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq;
+    const T geq = wi * (drho + rho * bracket);
     const T cf = cxi * fx + cyi * fy;
     const T src = wi * (T(3) * ((cxi - ux) * fx + (cyi - uy) * fy) + T(9) * cu * cf);
     g[i] = g[i] - omega * (g[i] - geq) + fp * src;
