@@ -97,11 +97,14 @@ int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* sing
  * operations per cell). block_x (64..512, multiple of 64; 0 keeps the current value) and rows
  * set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64, 48, 32, 24, 16
  * rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps the current
- * setting. lbm_step always advances one
+ * setting; tile_order (0, 1, 2 as in lbm_set_kernel_options, -1 keeps; default 1) orders its
+ * tiles over the XCDs. lbm_step always advances one
  * step. lbm_get_steps_per_launch reports the height in use and in_use whether the current
  * configuration takes the two-step path. */
-int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows);
+int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows,
+                             int tile_order);
 int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
+                             int* tile_order,
                              int* in_use);
 
 /* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */

@@ -71,8 +71,8 @@ def _declare(L):
         "lbm_set_force_field": ([vp, vp], i),
         "lbm_set_tau": ([vp, d], i),
         "lbm_set_block": ([vp, i, i], i),
-        "lbm_set_steps_per_launch": ([vp, i, i, i], i),
-        "lbm_get_steps_per_launch": ([vp, ip, ip, ip, ip], i),
+        "lbm_set_steps_per_launch": ([vp, i, i, i, i], i),
+        "lbm_get_steps_per_launch": ([vp, ip, ip, ip, ip, ip], i),
         "lbm_get_block": ([vp, ip, ip], i),
         "lbm_set_kernel_options": ([vp, i, i, i], i),
         "lbm_get_kernel_options": ([vp, ip, ip, ip], i),
@@ -370,19 +370,21 @@ class Solver:
     def set_block(self, block_x, rows_per_thread):
         check(lib().lbm_set_block(self._h, int(block_x), int(rows_per_thread)))
 
-    def set_steps_per_launch(self, steps_per_launch: int, block_x: int = 0, rows: int = 0):
+    def set_steps_per_launch(self, steps_per_launch: int, block_x: int = 0, rows: int = 0,
+                             tile_order: int = -1):
         """1: one step per launch; 2: run()/timed_run() advance pairs of steps in one launch of
         the two-step kernel where it applies (same bits). block_x: tile columns (0 keeps);
-        rows: tile rows, -1 automatic (up to 64, filling the GPU evenly), 0 keeps."""
+        rows: tile rows, -1 automatic (up to 64, filling the GPU evenly), 0 keeps; tile_order:
+        0 round-robin, 1 XCD bands (default), 2 XCD rows, -1 keeps."""
         check(lib().lbm_set_steps_per_launch(self._h, int(steps_per_launch), int(block_x),
-                                             int(rows)))
+                                             int(rows), int(tile_order)))
 
     def steps_per_launch(self) -> dict:
-        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        a, b, c, o, d = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         check(lib().lbm_get_steps_per_launch(self._h, C.byref(a), C.byref(b), C.byref(c),
-                                             C.byref(d)))
+                                             C.byref(o), C.byref(d)))
         return {"steps_per_launch": a.value, "block_x": b.value, "rows": c.value,
-                "in_use": bool(d.value)}
+                "tile_order": o.value, "in_use": bool(d.value)}
 
     def set_kernel_options(self, streaming_stores: bool, single_resource: bool,
                            tile_order: int = 0):

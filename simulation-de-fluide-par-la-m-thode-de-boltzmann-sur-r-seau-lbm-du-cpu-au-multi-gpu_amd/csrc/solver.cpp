@@ -228,8 +228,10 @@ int lbm_set_state(lbm_solver* s, const void* in) {
   return LBM_OK;
 }
 
-int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows) {
+int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows,
+                             int tile_order) {
   LBM_TRY(check_handle(s));
+  if (tile_order < -1 || tile_order > 2) return fail(LBM_EINVAL, "tile_order must be -1 (keep), 0, 1 or 2");
   if (steps_per_launch != 1 && steps_per_launch != 2)
     return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
   if (block_x != 0 && (block_x < 64 || block_x > 512 || block_x % 64))
@@ -241,12 +243,14 @@ int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, i
     s->rows2_auto = 0;
   }
   if (rows) s->rows2 = rows;
+  if (tile_order >= 0) s->two_step_order = tile_order;
   return LBM_OK;
 }
 
 int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
-                             int* in_use) {
+                             int* tile_order, int* in_use) {
   LBM_TRY(check_handle(s));
+  if (tile_order) *tile_order = s->two_step_order;
   if (steps_per_launch) *steps_per_launch = s->steps_per_launch;
   if (block_x) *block_x = s->block2_x;
   if (rows) *rows = s->two_step_rows();
@@ -271,12 +275,14 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
                   "\"gpu\": \"%s\", \"arch\": \"%s\", \"hip_runtime\": %d, \"hip_driver\": %d, "
                   "\"block\": \"%dx%d\", \"variant\": \"soa_pull_buffer\", \"storage\": \"%s\", "
                   "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\", "
-                  "\"plane_stride\": %zu, \"steps_per_launch\": %d, \"two_step_tile\": \"%dx%d\"",
+                  "\"plane_stride\": %zu, \"steps_per_launch\": %d, \"two_step_tile\": \"%dx%d\", "
+                  "\"two_step_order\": %d",
                   prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
                   s->xcd_bands == 1 ? "xcd_bands" : (s->xcd_bands == 2 ? "xcd_rows" : "round_robin"),
-                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->two_step_rows());
+                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->two_step_rows(),
+                  s->two_step_order);
   return LBM_OK;
 }
 

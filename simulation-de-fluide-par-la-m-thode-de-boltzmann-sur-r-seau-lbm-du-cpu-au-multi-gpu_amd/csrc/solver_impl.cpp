@@ -358,18 +358,20 @@ int SolverState::advance(int n) {
   if (n >= 2 && two_step_eligible()) {
     hipError_t e = hipSuccess;
     const int rows_tile = two_step_rows();
+    // XCD rows need whole groups of 8 tile rows; round-robin otherwise
+    const int order = two_step_order == 2 && ((ny + rows_tile - 1) / rows_tile) % 8 ? 0 : two_step_order;
     for (; n >= 2 && e == hipSuccess; n -= 2) {
       if (prec == LBM_FLOAT) {
         StepParams<float> p = make_step_params<float>(*this);
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
-        p.xcd_bands = two_step_order;
+        p.xcd_bands = order;
         e = launch_step2(p, block2_x, stream);
       } else {
         StepParams<double> p = make_step_params<double>(*this);
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
-        p.xcd_bands = two_step_order;
+        p.xcd_bands = order;
         e = launch_step2(p, block2_x, stream);
       }
       if (e == hipSuccess) {

@@ -387,3 +387,26 @@ def test_autotune_restores_state_and_keeps_results(lbm, prec):
         out.append(s.populations())
         s.close()
     assert_bitwise(out[0], out[1], "tuned vs untuned")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_two_step_tile_orders_match_oracle(lbm, oracle_mod, prec):
+    """The two-step kernel's tile orders (round-robin, XCD bands, XCD rows: 8 tile rows here)
+    give the oracle's bits."""
+    nx, ny = 640, 512
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[200:230, 500:520] = 1
+    solid = solid.reshape(-1)
+    o = oracle_mod.Oracle(nx, ny, 0.9, 3e-6, 1e-6, solid, prec, "v0")
+    o.init_equilibrium(1.0, 0.03, 0.0)
+    o.run(6)
+    ref = o.populations()
+    for order in (0, 1, 2):
+        s = lbm.Solver(nx, ny, 0.9, 3e-6, 1e-6, solid, prec)
+        s.set_steps_per_launch(2, 256, 64, order)
+        info = s.steps_per_launch()
+        assert info["in_use"] and info["tile_order"] == order
+        s.init_equilibrium(1.0, 0.03, 0.0)
+        s.run(6)
+        assert_bitwise(s.populations(), ref, f"two-step tile order {order}")
+        s.close()
