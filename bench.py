@@ -17,8 +17,9 @@ Besides the headline value the line carries:
   cpu_baseline  the reference's own v1 OpenMP solver compiled from its sources (oracle/_ref),
                 or the oracle's OpenMP port when that is absent, timed on this host on a
                 bounded sample of the same channel;
-  forms         the other measured forms (one step per launch, copy ceiling, fp64, FP16
-                storage, batched domains, strip share, adjoint cost, cylinder wake).
+  forms         the other measured forms (one step per launch, the reference's 5-repetition
+                protocol on the 8192^2 and 2048^2 instances, copy ceiling, fp64, FP16 storage,
+                batched domains, strip share, adjoint cost, cylinder wake).
 """
 from __future__ import annotations
 
@@ -103,6 +104,40 @@ def form_single_step(lbm, args):
             out.update(r)
         else:
             out["double"] = r
+    return out
+
+
+def reference_protocol(s, cells, steps, reps=5):
+    """The reference's timing protocol (src/common/driver.hpp:115-132, bench/run_bench.py:138-139):
+    untimed warm-up of steps / 5, then `reps` timed runs of `steps` steps in one process."""
+    import numpy as np
+
+    s.run(max(1, steps // 5))
+    m = np.array([cells * steps / s.timed_run(steps) / 1e6 for _ in range(reps)])
+    return {"mlups_mean": round(float(m.mean()), 1), "mlups_std": round(float(m.std()), 1),
+            "mlups_max": round(float(m.max()), 1), "reps": reps, "timed_steps": steps,
+            "warmup_steps": max(1, steps // 5)}
+
+
+def form_protocol(lbm, args):
+    """The headline instance and the config-0 throughput instance on the reference's protocol:
+    8192^2 channel (tau 0.6, gx 1e-6, 200 steps) and 2048^2 walls + seeded 8x8 block (tau 0.8,
+    F = (1e-5, 2e-6), rest init, 1000 steps), fp32, default path."""
+    out = {}
+    s = lbm.Solver(args.nx, args.ny, 0.6, 1e-6, 0.0, lbm.make_channel(args.nx, args.ny), "float")
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    out["channel_8192"] = reference_protocol(s, args.nx * args.ny, 200)
+    s.close()
+    n = 2048
+    s = lbm.Solver(n, n, 0.8, 1e-5, 2e-6, lbm.seeded_block_mask(0, n, n), "float")
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    out["block_2048"] = reference_protocol(s, n * n, 1000)
+    out["block_2048"]["block_seed"] = 0
+    out["block_2048"]["block_xy"] = list(lbm.seeded_block_position(0, n, n))
+    out["block_2048"]["note"] = "fp32 pair 302 MB: partly held by the 256 MB Infinity Cache"
+    s.close()
+    log(f"protocol: 8192^2 {out['channel_8192']['mlups_mean']:.0f} MLUPS, 2048^2 block "
+        f"{out['block_2048']['mlups_mean']:.0f} MLUPS (mean of 5)")
     return out
 
 
@@ -249,7 +284,8 @@ def form_cylinder(lbm, args):
     return out
 
 
-FORMS = {"single_step": form_single_step, "copy_ceiling": form_copy_ceiling,
+FORMS = {"single_step": form_single_step, "protocol": form_protocol,
+         "copy_ceiling": form_copy_ceiling,
          "precisions": form_precisions,
          "batched": form_batched, "strip": form_strip, "adjoint": form_adjoint,
          "cylinder": form_cylinder}

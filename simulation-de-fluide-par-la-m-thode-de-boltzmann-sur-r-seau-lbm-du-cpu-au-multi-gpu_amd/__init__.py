@@ -210,6 +210,24 @@ def make_cylinder(nx: int, ny: int, xc: float, yc: float, D: float) -> np.ndarra
     return (dx * dx + dy * dy <= 0.25 * D * D).astype(np.uint8).reshape(-1)
 
 
+def seeded_block_position(seed: int, nx: int, ny: int, size: int = 8) -> tuple[int, int]:
+    """Lower-left corner of the seeded square obstacle (the reference has no RNG; this is the
+    build's seed -> position map): numpy default_rng(seed), x0 uniform in [0, nx - size], y0
+    uniform in [1, ny - size - 1] (clear of the walls)."""
+    rng = np.random.default_rng(seed)
+    x0 = int(rng.integers(0, nx - size + 1))
+    y0 = int(rng.integers(1, ny - size))
+    return x0, y0
+
+
+def seeded_block_mask(seed: int, nx: int, ny: int, size: int = 8) -> np.ndarray:
+    """Walled channel plus a size x size solid block placed by `seed`."""
+    m = make_channel(nx, ny).reshape(ny, nx)
+    x0, y0 = seeded_block_position(seed, nx, ny, size)
+    m[y0:y0 + size, x0:x0 + size] = 1
+    return m.reshape(-1)
+
+
 def add_disc(mask: np.ndarray, nx: int, ny: int, xc: float, yc: float, r: float) -> np.ndarray:
     """Adds a staircase disc of radius r to a flat mask (in place) and returns it."""
     return np.maximum(mask, make_cylinder(nx, ny, xc, yc, 2.0 * r), out=mask)

@@ -99,3 +99,16 @@ def test_strouhal_recovers_a_known_frequency(lbm):
     assert abs(st_fft - exact) / exact < 1e-3
     assert abs(st_zero - exact) / exact < 1e-3
     assert n > 100
+
+
+def test_seeded_block_matches_the_fixture_generator(lbm):
+    """The package's seeded obstacle is the one the committed fixtures were written with."""
+    import importlib.util
+    from conftest import ROOT
+
+    spec = importlib.util.spec_from_file_location("make_golden", ROOT / "tests" / "golden" / "make_golden.py")
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    for seed in (0, 1, 7, 123):
+        assert lbm.seeded_block_position(seed, 53, 31) == mg.seeded_block_position(seed)
+        assert (lbm.seeded_block_mask(seed, 53, 31) == mg.seeded_block_mask(seed)).all()
