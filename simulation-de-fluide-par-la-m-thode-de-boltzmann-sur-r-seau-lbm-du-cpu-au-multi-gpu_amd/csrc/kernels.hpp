@@ -163,13 +163,20 @@ constexpr size_t kPlaneTail = 64;
 // launch_step (solid cells keep their values in both).
 template <class T>
 hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream);
-// LDS bytes per workgroup of launch_step2 (independent of the tile height).
+// LDS bytes per workgroup of launch_step2 (independent of the tile height); elem = bytes of the
+// computed type (4 for float and FP16 storage, 8 for double).
 size_t step2_lds_bytes(int block_x, size_t elem);
 // Tile height for launch_step2: the tallest of 64, 48, 32, 24, 16 rows whose tile count fills
 // the resident workgroup slots (occupancy of the instance p selects, on the current device)
 // evenly, else the best-filling one.
 template <class T>
 int step2_tile_rows(const StepParams<T>& p, int block_x);
+
+// FP16 storage, two steps per launch (step2.hip): the conditions of launch_step2; p.seed is the
+// stochastic-rounding seed of the first step, seed2 of the second. Bit-identical to two
+// launch_step_fp16 steps. LDS per workgroup: step2_lds_bytes(block_x, sizeof(float)).
+hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream);
+int step2_tile_rows_fp16(const StepParamsH& p, int block_x);
 
 // Strip ring closed on itself: ghost rows 0 and nyl + 1 of the state g (plane stride N, rows
 // of nx values) refreshed from the owned boundary rows, one launch.

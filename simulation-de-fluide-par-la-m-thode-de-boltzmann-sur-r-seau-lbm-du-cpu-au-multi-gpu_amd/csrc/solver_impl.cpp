@@ -108,11 +108,18 @@ StepParamsH make_step_params_h(const SolverState& s) {
   float eq[kQ] = {};
   if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i] * p.scale;
-  p.seed = s.step_count * 0x9E3779B9u + 0x7F4A7C15u;
+  p.seed = fp16_seed(s.step_count);
   p.xcd_bands = s.tile_order_for(s.rows);
   p.col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
   return p;
 }
+
+}  // namespace
+
+// Stochastic-rounding seed of FP16 step number `step`.
+uint32_t fp16_seed(uint32_t step) { return step * 0x9E3779B9u + 0x7F4A7C15u; }
+
+namespace {
 
 // float -> scaled half, round to nearest even (host side, for uploads).
 uint16_t to_half(float f) {
@@ -337,9 +344,10 @@ int SolverState::step() {
 }
 
 bool SolverState::two_step_eligible() const {
-  if (steps_per_launch != 2 || open_x || force || prec == LBM_FP16) return false;
+  if (steps_per_launch != 2 || open_x || force) return false;
   if (S * storage_size(prec) >= (size_t(1) << 32)) return false;
-  return step2_lds_bytes(block2_x, storage_size(prec)) <= 64 * 1024;
+  // the LDS ring holds computed values: fp32 for FP16 storage
+  return step2_lds_bytes(block2_x, prec == LBM_DOUBLE ? 8 : 4) <= 64 * 1024;
 }
 
 int SolverState::two_step_rows() const {
@@ -347,6 +355,8 @@ int SolverState::two_step_rows() const {
   if (!rows2_auto) {
     if (prec == LBM_DOUBLE) {
       rows2_auto = step2_tile_rows(make_step_params<double>(*this), block2_x);
+    } else if (prec == LBM_FP16) {
+      rows2_auto = step2_tile_rows_fp16(make_step_params_h(*this), block2_x);
     } else {
       rows2_auto = step2_tile_rows(make_step_params<float>(*this), block2_x);
     }
@@ -361,7 +371,13 @@ int SolverState::advance(int n) {
     // XCD rows need whole groups of 8 tile rows; round-robin otherwise
     const int order = two_step_order == 2 && ((ny + rows_tile - 1) / rows_tile) % 8 ? 0 : two_step_order;
     for (; n >= 2 && e == hipSuccess; n -= 2) {
-      if (prec == LBM_FLOAT) {
+      if (prec == LBM_FP16) {
+        StepParamsH p = make_step_params_h(*this);  // seed of step step_count
+        p.rows_per_tile = rows_tile;
+        p.tiles_x = (nx + block2_x - 1) / block2_x;
+        p.xcd_bands = order;
+        e = launch_step2_fp16(p, fp16_seed(step_count + 1), block2_x, stream);
+      } else if (prec == LBM_FLOAT) {
         StepParams<float> p = make_step_params<float>(*this);
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;

@@ -1,4 +1,5 @@
-// Two fused time steps per launch (temporal blocking), periodic domain with a uniform force:
+// Two fused time steps per launch (temporal blocking), periodic domain with a uniform force
+// (float, double, and FP16 storage with its own cell update and rounding):
 // R_{t+2} from R_t with every population read once from HBM and written once, i.e. 72 B (fp32)
 // per two cell updates instead of per one. Each cell update performs exactly the operations of
 // step_kernel (the same pull, the same bounce-back rule, the same collide()), so the result is
@@ -28,6 +29,7 @@
 #include <type_traits>
 
 #include "batch.hpp"
+#include "half_cell.hpp"
 #include "kernels.hpp"
 #include "step_body.hpp"
 
@@ -90,39 +92,89 @@ __device__ __forceinline__ S load_uniform(const S* ptr) {
 }
 
 // One periodic domain with a uniform force: the plain solver's grid or one batched domain.
-template <class T>
+// S is the stored population type.
+template <class S>
 struct Step2View {
-  const T* g;
-  T* gn;
+  const S* g;
+  S* gn;
   const uint16_t* links;
   const uint8_t* segflags;
   size_t N;  // plane stride, >= nx * ny + kPlaneTail
   int nx, ny, nseg;
+};
+
+// Cell updates of the walk. Stored type S, computed type C, written word Word; step1() turns a
+// cell's pulled values into its state one step later as the next step will read it, step2()
+// into the words the store writes. `cell` is the cell's index y * nx + x.
+// float / double: the reference's collision, bit-identical to step_kernel.
+template <class T>
+struct PlainCells {
+  using S = T;
+  using C = T;
+  using Word = T;
   T omega, fp, fx, fy;
+  __device__ __forceinline__ void step1(C (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
+  __device__ __forceinline__ void step2(C (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
+    collide(v, fx, fy, omega, fp);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) out[i] = v[i];
+  }
+  template <int kAux, class PS>
+  __device__ __forceinline__ static void store(const PS& gn, int i, unsigned vo, unsigned so, Word w) {
+    gn.template store<T, kAux>(i, vo, so, w);
+  }
+};
+
+// FP16 storage: the collision and the stochastic rounding of step_fp16_kernel, with the seed of
+// each of the two steps, so the result is bit-identical to two one-step launches; step 2 reads
+// the rounded step-1 values, as the second launch would.
+struct HalfCells {
+  using S = uint16_t;
+  using C = float;
+  using Word = unsigned short;
+  HalfCollide col;
+  uint32_t seed1, seed2;
+  __device__ __forceinline__ void step1(C (&v)[kQ], uint32_t cell) const {
+    fast_collide(col, v);
+    unsigned short h[kQ];
+    round_cell(v, hash_cell(cell, seed1), h);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = half_value(h[i]);
+  }
+  __device__ __forceinline__ void step2(C (&v)[kQ], uint32_t cell, Word (&out)[kQ]) const {
+    fast_collide(col, v);
+    round_cell(v, hash_cell(cell, seed2), out);
+  }
+  template <int, class PS>
+  __device__ __forceinline__ static void store(const PS& gn, int i, unsigned vo, unsigned so, Word w) {
+    gn.store_bits(i, vo, so, w);
+  }
 };
 
 // Step 1 at a halo-column cell (c, r) (both wrapped) whose row differs between the lanes of a
 // wave: the row offsets go into the per-lane offset (a per-lane scalar offset would make the
 // compiler loop over its values). The pull with its bounce-back corrections and the collision,
 // exactly as update_column.
-template <class T, class PS>
-__device__ __forceinline__ void halo_step(const Step2View<T>& p, const PS& g, int c, int r,
-                                          T (&v)[kQ]) {
-  constexpr unsigned B = sizeof(T);
+template <class P, class PS>
+__device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, const P& cells,
+                                          const PS& g, int c, int r, typename P::C (&v)[kQ]) {
+  using S = typename P::S;
+  constexpr unsigned B = sizeof(S);
   const Rows rows = rows_of(r, p.nx, p.ny, B);
   const Cols cols = cols_of(c, p.nx, B);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) v[i] = g.template load<T>(i, cols.of(kCx[i]) + rows.of(kCy[i]), 0u);
+  for (int i = 0; i < kQ; ++i) v[i] = g.template load<S>(i, cols.of(kCx[i]) + rows.of(kCy[i]), 0u);
   uint32_t link = 0;
   if (p.segflags[static_cast<size_t>(r) * p.nseg + (c >> 6)]) {
     link = p.links[static_cast<size_t>(r) * p.nx + c];
     if (!(link & kLinkSolid)) {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        if (link & (1u << i)) v[i] = g.template load<T>(kOpp[i], cols.c + rows.c, 0u);
+        if (link & (1u << i)) v[i] = g.template load<S>(kOpp[i], cols.c + rows.c, 0u);
     }
   }
-  if (!(link & kLinkSolid)) collide(v, p.fx, p.fy, p.omega, p.fp);
+  if (!(link & kLinkSolid))
+    cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(p.nx) + static_cast<uint32_t>(c));
 }
 
 // Halo-column cells are computed for kHaloChunk step-1 rows at a time (the first chunk also
@@ -133,9 +185,12 @@ constexpr int kHaloRows = kHaloChunk + 2;
 // The walk of one tile: columns x0 .. x0 + blockDim.x - 1 (those inside the grid), output rows
 // y0 .. y0 + H - 1 (those inside the grid). Ends with every LDS read done before the next walk's
 // first LDS write crosses a barrier (its halo chunk ends with one).
-template <class T, class PS, int kStoreAux>
-__device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, const int y0,
-                                           const int H, unsigned char* smem) {
+template <class P, class PS, int kStoreAux>
+__device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, const P& cells,
+                                           const int x0, const int y0, const int H,
+                                           unsigned char* smem) {
+  using T = typename P::C;  // registers and LDS hold computed values
+  using S = typename P::S;
   const int W = static_cast<int>(blockDim.x), WP = W + 2;
   T* ring = reinterpret_cast<T*>(smem);                  // [kRingSlots][kRingPlanes][W + 2]
   T* halo = ring + kRingSlots * kRingPlanes * WP;        // [2][kHaloRows][3]
@@ -159,7 +214,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
       if (j < 2 * cnt) {
         const int side = j >= cnt ? 1 : 0, kk = j - side * cnt;
         T v[kQ];
-        halo_step(p, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v);
+        halo_step(p, cells, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v);
         T* h = halo + (side * kHaloRows + kk) * 3;
 #pragma unroll
         for (int q = 0; q < 3; ++q) h[q] = v[side ? kRightPlanes[q] : kLeftPlanes[q]];
@@ -173,7 +228,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
   // column included: they compute its values and store them to its address, the same bits.
   const int tt = t < wv ? t : wv - 1;
   const int c = x0 + tt;
-  constexpr unsigned B = sizeof(T);
+  constexpr unsigned B = sizeof(S);
   const unsigned vo = static_cast<unsigned>(c) * B;
   // A solid cell is not written: its step-2 result goes to a discard slot past the plane's
   // last cell (a per-lane offset instead of a branch around the stores, see below).
@@ -185,7 +240,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
 #pragma unroll
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
   T cur[kQ];            // raw pulls of the current step-1 row
-  pull_streamed<T>(g, uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cols, cur);
+  pull_streamed<S>(g, uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cols, cur);
 
   // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
   // are issued first and nothing between them and their use branches around a vector memory
@@ -197,7 +252,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
     const int r = wrap_y(y0 - 1 + k);
     const Rows rows = uniform_rows_of(r, nx, ny, B);
     T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
-    pull_streamed<T>(g, uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), cols, nxt);
+    pull_streamed<S>(g, uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), cols, nxt);
     // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
     // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
@@ -208,11 +263,11 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
       link = p.links[static_cast<size_t>(r) * nx + c];
       T own[kQ];
 #pragma unroll
-      for (int i = 1; i < kQ; ++i) own[i] = g.template load<T>(kOpp[i], cols.c, rows.c);
+      for (int i = 1; i < kQ; ++i) own[i] = g.template load<S>(kOpp[i], cols.c, rows.c);
 #pragma unroll
       for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? own[i] : v[i];
     }
-    collide(v, p.fx, p.fy, p.omega, p.fp);
+    cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c));
     T* sl = ring + slot * kRingPlanes * WP;
 #pragma unroll
     for (int i : {1, 3, 5, 6, 7, 8}) sl[ring_index(i) * WP + tt + 1] = v[i];
@@ -249,12 +304,14 @@ __device__ __forceinline__ void step2_walk(const Step2View<T>& p, const int x0, 
         for (int i = 1; i < kQ; ++i)
           if (bb & (1u << i)) w[i] = prev[kOpp[i]];
       }
-      collide(w, p.fx, p.fy, p.omega, p.fp);
+      typename P::Word out[kQ];
+      const int q = y0 - 2 + k;
+      cells.step2(w, static_cast<uint32_t>(q) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c), out);
       const unsigned so = static_cast<unsigned>(
-          __builtin_amdgcn_readfirstlane(static_cast<unsigned>((y0 - 2 + k) * nx) * B));
+          __builtin_amdgcn_readfirstlane(static_cast<unsigned>(q * nx) * B));
       const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
 #pragma unroll
-      for (int i = 0; i < kQ; ++i) gn.template store<T, kStoreAux>(i, vs, so, w[i]);
+      for (int i = 0; i < kQ; ++i) P::template store<kStoreAux>(gn, i, vs, so, out[i]);
     }
     prev2 = prev[2];
 #pragma unroll
@@ -300,10 +357,24 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
   const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
-  const Step2View<T> d{p.g,  p.gn,   p.links, p.segflags, p.N,  p.nx,
-                       p.ny, p.nseg, p.omega, p.fp,       p.fx, p.fy};
-  step2_walk<T, PS, kStoreAux>(d, tx * static_cast<int>(blockDim.x), ty * p.rows_per_tile,
-                               p.rows_per_tile, smem);
+  const Step2View<T> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
+  const PlainCells<T> cells{p.omega, p.fp, p.fx, p.fy};
+  step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tx * static_cast<int>(blockDim.x),
+                                           ty * p.rows_per_tile, p.rows_per_tile, smem);
+}
+
+// FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
+template <class PS>
+__global__ void __launch_bounds__(512) step2_fp16_kernel(const StepParamsH p, const uint32_t seed2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x,
+                                      static_cast<unsigned>(p.tiles_x));
+  const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
+  const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
+  const Step2View<uint16_t> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
+  const HalfCells cells{p.col, p.seed, seed2};
+  step2_walk<HalfCells, PS, kAuxNone>(d, cells, tx * static_cast<int>(blockDim.x),
+                                      ty * p.rows_per_tile, p.rows_per_tile, smem);
 }
 
 // Batched domains: workgroup b walks tile b of the two-step work-list (domain, column tile, row
@@ -314,10 +385,10 @@ __global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p
   const BatchTile tile = load_uniform(p.tiles + blockIdx.x);
   const BatchDomain<T> dm = load_uniform(p.domains + tile.domain);
   const Step2View<T> d{p.g + dm.offset, p.gn + dm.offset, p.links + dm.cell_offset,
-                       p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg,
-                       dm.omega, dm.fp, dm.fx, dm.fy};
-  step2_walk<T, PS, kStoreAux>(d, tile.tx * static_cast<int>(blockDim.x), tile.ty * p.tile_rows,
-                               p.tile_rows, smem);
+                       p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg};
+  const PlainCells<T> cells{dm.omega, dm.fp, dm.fx, dm.fy};
+  step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tile.tx * static_cast<int>(blockDim.x),
+                                           tile.ty * p.tile_rows, p.tile_rows, smem);
 }
 
 template <class T, class PS>
@@ -384,18 +455,13 @@ hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, hipStream_t 
 template hipError_t launch_batch_step2<float>(const BatchParams<float>&, int, hipStream_t);
 template hipError_t launch_batch_step2<double>(const BatchParams<double>&, int, hipStream_t);
 
-template <class T>
-int step2_tile_rows(const StepParams<T>& p, int block_x) {
+// Tile height for a two-step kernel instance: `kernel` with block_x threads and `lds` bytes.
+template <class K>
+int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds) {
   // Workgroups of this instance resident at once on the current device.
-  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
-  const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
   int per_cu = 0, dev = 0, cus = 0;
-  const hipError_t e =
-      one ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, PlanesOne, kAuxNt>,
-                                                         block_x, lds)
-          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step2_kernel<T, Planes, kAuxNt>,
-                                                         block_x, lds);
-  if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_x, lds) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       per_cu < 1 || cus < 1) {
     (void)hipGetLastError();
@@ -407,11 +473,11 @@ int step2_tile_rows(const StepParams<T>& p, int block_x) {
   // last wave 56 % full) 125.8k, 56 rows 128.3k; 32 rows (eight waves, more recomputed rows)
   // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k.
   const long slots = static_cast<long>(per_cu) * cus;
-  const long tiles_x = (p.nx + block_x - 1) / block_x;
+  const long tiles_x = (nx + block_x - 1) / block_x;
   int best = 16;
   double best_fill = -1.0;
   for (const int rows : {64, 48, 32, 24, 16}) {
-    const long tiles = tiles_x * ((p.ny + rows - 1) / rows);
+    const long tiles = tiles_x * ((ny + rows - 1) / rows);
     const double fill = static_cast<double>(tiles) / (((tiles + slots - 1) / slots) * slots);
     if (fill >= 0.97) return rows;
     if (fill > best_fill) {
@@ -421,7 +487,48 @@ int step2_tile_rows(const StepParams<T>& p, int block_x) {
   }
   return best;
 }
+
+template <class T>
+int step2_tile_rows(const StepParams<T>& p, int block_x) {
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
+  const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
+  return one ? tile_rows_for(step2_kernel<T, PlanesOne, kAuxNt>, p.nx, p.ny, block_x, lds)
+             : tile_rows_for(step2_kernel<T, Planes, kAuxNt>, p.nx, p.ny, block_x, lds);
+}
 template int step2_tile_rows<float>(const StepParams<float>&, int);
 template int step2_tile_rows<double>(const StepParams<double>&, int);
+
+namespace {
+bool fp16_one_resource(const StepParamsH& p) {
+  return static_cast<uint64_t>(kQ) * p.N * sizeof(uint16_t) < (uint64_t{1} << 32);
+}
+}  // namespace
+
+hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream) {
+  // periodic, uniform force, whole domain, tile rows >= 1, planes addressable with 32 bits and
+  // holding the discard slots
+  if (!p.wrap_y || p.force || p.row0 != 0 || p.nrows != p.ny || p.rows_per_tile < 1 ||
+      block_x < 64 || block_x > 512 || block_x % 64 ||
+      p.tiles_x != (p.nx + block_x - 1) / block_x ||
+      static_cast<uint64_t>(p.N) * sizeof(uint16_t) >= (uint64_t{1} << 32) ||
+      p.N < static_cast<size_t>(p.nx) * p.ny + kPlaneTail)
+    return hipErrorInvalidValue;
+  const size_t lds = step2_lds_bytes(block_x, sizeof(float));
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const unsigned tiles_y = static_cast<unsigned>((p.ny + p.rows_per_tile - 1) / p.rows_per_tile);
+  if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
+  const unsigned grid = static_cast<unsigned>(p.tiles_x) * tiles_y;
+  if (fp16_one_resource(p))
+    step2_fp16_kernel<PlanesOne><<<grid, block_x, lds, stream>>>(p, seed2);
+  else
+    step2_fp16_kernel<Planes><<<grid, block_x, lds, stream>>>(p, seed2);
+  return hipGetLastError();
+}
+
+int step2_tile_rows_fp16(const StepParamsH& p, int block_x) {
+  const size_t lds = step2_lds_bytes(block_x, sizeof(float));
+  return fp16_one_resource(p) ? tile_rows_for(step2_fp16_kernel<PlanesOne>, p.nx, p.ny, block_x, lds)
+                              : tile_rows_for(step2_fp16_kernel<Planes>, p.nx, p.ny, block_x, lds);
+}
 
 }  // namespace lbmhip

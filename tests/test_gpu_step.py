@@ -22,6 +22,8 @@ SHAPES = [(64, 1), (64, 3), (128, 8), (192, 5), (256, 8), (512, 2), (1024, 1)]
 
 def bits(a):
     a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:  # raw FP16 storage
+        return a
     return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
 
 
@@ -354,7 +356,7 @@ def test_two_step_path_is_the_default_where_it_applies(lbm):
     s.set_inflow_outflow(0.01)
     assert not s.steps_per_launch()["in_use"]   # open x: single steps
     s2 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "fp16")
-    assert not s2.steps_per_launch()["in_use"]  # FP16 storage: single steps
+    assert s2.steps_per_launch()["in_use"]  # FP16 storage too
     s3 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "double")
     s3.set_force_field(np.zeros(2 * 256 * 64))
     assert not s3.steps_per_launch()["in_use"]  # per-cell force field: single steps
@@ -410,3 +412,33 @@ def test_two_step_tile_orders_match_oracle(lbm, oracle_mod, prec):
         s.run(6)
         assert_bitwise(s.populations(), ref, f"two-step tile order {order}")
         s.close()
+
+
+@pytest.mark.parametrize("grid,tile", [((53, 31), (64, 7)), ((300, 90), (512, -1)),
+                                       ((640, 416), (256, 64)), ((333, 700), (128, 100))])
+def test_fp16_two_step_equals_one_step_launches(lbm, grid, tile):
+    """FP16 storage with two steps per launch: each step's stochastic rounding (its own seed,
+    the cell's own index) applied to the step-1 values before step 2 reads them, so the raw
+    state equals the one-step kernel's bit for bit after even and odd step counts; solid cells
+    keep their values."""
+    nx, ny = grid
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[ny // 3: ny // 3 + 9, min(nx - 12, 40): min(nx - 12, 40) + 7] = 1
+    solid = solid.reshape(-1)
+    rng = np.random.default_rng(nx + 3 * ny)
+    S = (rng.standard_normal(nx * ny * 9) * 2e-3).astype(np.float32)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    out = []
+    for spl in (2, 1):
+        s = lbm.Solver(nx, ny, 0.7, 2e-5, -5e-6, solid, "fp16")
+        s.set_steps_per_launch(spl, *tile)
+        assert s.steps_per_launch()["in_use"] == (spl == 2)
+        s.set_populations(S)
+        states = []
+        for n in (2, 1, 3, 6):
+            s.run(n)
+            states.append(s.get_state())
+        out.append(states)
+        s.close()
+    for k, (a, b) in enumerate(zip(*out)):
+        assert_bitwise(a, b, f"FP16 {grid} tile {tile} after run {k}")
