@@ -92,7 +92,7 @@ int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* sing
 
 /* No reference counterpart: temporal blocking. With steps_per_launch = 2 (the default)
  * lbm_run and lbm_timed_run advance pairs of steps in one launch of a two-step kernel wherever
- * it applies (periodic domain, uniform force, float or double): every population is read from
+ * it applies (periodic domain, uniform force; float, double or FP16 storage): every population is read from
  * HBM and written once per two steps. The results are bit-identical to single steps (same
  * operations per cell). block_x (64..512, multiple of 64; 0 keeps the current value) and rows
  * set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64, 48, 32, 24, 16
