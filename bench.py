@@ -13,7 +13,10 @@ Besides the headline value the line carries:
                 fp32 populations read once and written once for two updates) x cells / average
                 launch time, against the 8 TB/s HBM peak; `traffic` is the rocprofv3-measured
                 HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-                calibration), read from the committed profile summary;
+                calibration), read from the committed profile summary; `valu_issue_frac` the
+                profiled VALU instructions of a launch (SQ_INSTS_VALU, same summary) against
+                the vector-ALU issue ceiling (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
+                instruction) over the measured launch time: the kernel's other bound;
   cpu_baseline  the reference's own v1 OpenMP solver compiled from its sources (oracle/_ref),
                 or the oracle's OpenMP port when that is absent, timed on this host on a
                 bounded sample of the same channel;
@@ -45,12 +48,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the fp32 two-step kernel, from the committed rocprofv3 summary."""
+def from_profile(key):
+    """A per-launch figure of the fp32 two-step kernel from the committed rocprofv3 summary:
+    traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE, calibrated) or
+    valu_instructions_per_launch (SQ_INSTS_VALU)."""
     if not PROFILE_SUMMARY.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", PROFILE_SUMMARY.read_text())
+    m = re.search(key + r":\s*([0-9.eE+]+)", PROFILE_SUMMARY.read_text())
     return float(m.group(1)) if m else None
+
+
+# Vector-ALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
+# 4 cycles at the 2.4 GHz peak engine clock.
+VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def run_plain(lbm, nx, ny, steps, warmup, precision="float", block=None, steps_per_launch=2):
@@ -364,12 +374,18 @@ def main():
                    "solver": solver},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(),
+                     "traffic": from_profile("traffic_bytes_per_launch"),
                      "kernel": "step2_kernel<float>" if per_launch == 2 else "step_kernel<float>",
                      "updates_per_launch": per_launch,
                      "bytes_per_launch": BYTES_PER_CELL["float"] * cells,
                      "launch_ms": round(launch_s * 1e3, 5)},
     }
+    valu = from_profile("valu_instructions_per_launch")
+    if valu and per_launch == 2:
+        # the two-step kernel is bound by vector-ALU issue as much as by HBM: the profiled VALU
+        # instruction count of a launch against the issue ceiling, over the launch time measured here
+        line["roofline"]["valu_issue_frac"] = round(valu / VALU_WAVE_INSTR_PER_S / launch_s, 4)
+        line["roofline"]["valu_instructions_per_launch"] = valu
     if not args.no_forms:
         line["forms"] = forms(lbm, args)
         f = line["forms"]
