@@ -215,6 +215,7 @@ def form_batched(lbm, args):
     return {"domains": 256, "seed": 0, "cells": cells, "mlups": round(mlups, 1),
             "timed_steps": 1000, "graph_steps": 100, "steps_per_launch": per_launch,
             "two_step_tile": f"{two['tile_x']}x{two['tile_rows']}",
+            "two_step_workgroups": two["workgroups"],
             "one_step": {"order": order, "streaming_stores": nt, "tile_x": tx, "tile_rows": tr,
                          "workgroups": wg},
             "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / per_launch / 1e9, 1)}

@@ -213,12 +213,14 @@ int lbm_batch_set_order(lbm_batch* b, int order, int streaming_stores);
 int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores);
 /* Temporal blocking over the batch (every domain is periodic with a uniform force): with
  * steps_per_launch = 2 (the default) lbm_batch_run / lbm_batch_timed_run and their graphs
- * advance pairs of steps in one launch of the two-step kernel, one workgroup per tile of tile_x
- * columns (64..512, multiple of 64; double at most 256) by tile_rows rows (0 keeps either);
- * an odd count ends with one single step. Same bits as single steps. */
-int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows);
+ * advance pairs of steps in one launch of the two-step kernel over a list of tiles of tile_x
+ * columns (64..512, multiple of 64; double at most 256) by tile_rows rows (0 keeps either),
+ * walked by `workgroups` persistent workgroups (0, the default: one per tile; -1 keeps); an
+ * odd count ends with one single step. Same bits as single steps. */
+int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows,
+                                   int workgroups);
 int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, int* tile_x,
-                                   int* tile_rows);
+                                   int* tile_rows, int* workgroups);
 int lbm_batch_run(lbm_batch* b, int steps);
 int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds);
 int lbm_batch_populations(lbm_batch* b, int domain, void* out);

@@ -118,8 +118,8 @@ def _declare(L):
         "lbm_batch_autotune": ([vp, i, ip, ip, ip, dp], i),
         "lbm_batch_set_order": ([vp, i, i], i),
         "lbm_batch_get_order": ([vp, ip, ip], i),
-        "lbm_batch_set_steps_per_launch": ([vp, i, i, i], i),
-        "lbm_batch_get_steps_per_launch": ([vp, ip, ip, ip], i),
+        "lbm_batch_set_steps_per_launch": ([vp, i, i, i, i], i),
+        "lbm_batch_get_steps_per_launch": ([vp, ip, ip, ip, ip], i),
         "lbm_adjoint_create": ([i, i, d, _u8p, i, i, i, i, i, i, i, i, C.POINTER(vp)], i),
         "lbm_adjoint_destroy": ([vp], i),
         "lbm_adjoint_set_force_field": ([vp, vp], i),
@@ -603,16 +603,20 @@ class Batch:
         check(lib().lbm_batch_total_cells(self._h, C.byref(n)))
         return n.value
 
-    def set_steps_per_launch(self, steps_per_launch: int, tile_x: int = 0, tile_rows: int = 0):
+    def set_steps_per_launch(self, steps_per_launch: int, tile_x: int = 0, tile_rows: int = 0,
+                             workgroups: int = -1):
         """1: one step per launch; 2 (default): pairs of steps in one two-step launch (same
-        bits); tile_x / tile_rows set its tiles (0 keeps)."""
+        bits); tile_x / tile_rows set its tiles (0 keeps); workgroups its persistent grid (0:
+        one workgroup per tile, -1 keeps)."""
         check(lib().lbm_batch_set_steps_per_launch(self._h, int(steps_per_launch), int(tile_x),
-                                                   int(tile_rows)))
+                                                   int(tile_rows), int(workgroups)))
 
     def steps_per_launch(self) -> dict:
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        check(lib().lbm_batch_get_steps_per_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"steps_per_launch": a.value, "tile_x": b.value, "tile_rows": c.value}
+        a, b, c, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().lbm_batch_get_steps_per_launch(self._h, C.byref(a), C.byref(b), C.byref(c),
+                                                   C.byref(w)))
+        return {"steps_per_launch": a.value, "tile_x": b.value, "tile_rows": c.value,
+                "workgroups": w.value}
 
     def autotune(self, steps=10):
         tx, tr, wg, m = C.c_int(), C.c_int(), C.c_int(), C.c_double()

@@ -55,9 +55,11 @@ template <class T>
 hipError_t launch_batch(const BatchParams<T>& p, int block_x, int workgroups, hipStream_t stream);
 
 // Two steps per launch over the batch (step2.hip): p.tiles is the two-step work-list (tiles of
-// block_x columns by p.tile_rows rows), one workgroup per tile; every domain's plane stride
-// holds kPlaneTail values past its cells. Bit-identical to two launch_batch steps.
+// block_x columns by p.tile_rows rows), walked by `workgroups` persistent workgroups (0: one per
+// tile); every domain's plane stride holds kPlaneTail values past its cells. Bit-identical to
+// two launch_batch steps.
 template <class T>
-hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, hipStream_t stream);
+hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, int workgroups,
+                              hipStream_t stream);
 
 }  // namespace lbmhip

@@ -51,6 +51,7 @@ struct lbm_batch {
   // two steps per launch: its own tile list (tile2_x columns by tile2_rows rows)
   int steps_per_launch = 2;
   int tile2_x = 256, tile2_rows = 32;
+  int workgroups2 = 0;  // persistent grid of the two-step kernel (0: one workgroup per tile)
   BatchTile* dtiles2 = nullptr;
   size_t tiles2_cap = 0;
   int ntiles2 = 0;
@@ -152,7 +153,9 @@ struct lbm_batch {
     return LBM_OK;
   }
 
-  // Two-step work-list: tiles of tile2_x columns by tile2_rows rows, domain by domain.
+  // Two-step work-list: tiles of tile2_x columns by tile2_rows rows, domain by domain, row by
+  // row, so that the tiles running together are neighbours sharing halo rows and edge lines in
+  // L2 (a largest-first order measured 14-17 % slower on the 256 seeded domains).
   int build_tiles2() {
     std::vector<BatchTile> t;
     for (size_t k = 0; k < doms.size(); ++k) {
@@ -188,8 +191,9 @@ struct lbm_batch {
   }
 
   int launch2(bool flip) {
-    const hipError_t e = prec == LBM_DOUBLE ? launch_batch_step2(params2<double>(flip), tile2_x, stream)
-                                            : launch_batch_step2(params2<float>(flip), tile2_x, stream);
+    const hipError_t e = prec == LBM_DOUBLE
+                             ? launch_batch_step2(params2<double>(flip), tile2_x, workgroups2, stream)
+                             : launch_batch_step2(params2<float>(flip), tile2_x, workgroups2, stream);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("two-step batch launch: ") + hipGetErrorString(e));
     return LBM_OK;
   }
@@ -424,7 +428,8 @@ int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores) {
   return LBM_OK;
 }
 
-int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows) {
+int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows,
+                                   int workgroups) {
   if (!b) return fail(LBM_EINVAL, "null batch handle");
   if (steps_per_launch != 1 && steps_per_launch != 2)
     return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
@@ -433,20 +438,23 @@ int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_
   if (tile_x != 0 && step2_lds_bytes(tile_x, b->rs()) > 64 * 1024)
     return fail(LBM_EINVAL, "two-step tile too wide for this precision (LDS)");
   if (tile_rows < 0) return fail(LBM_EINVAL, "tile_rows must be >= 0");
+  if (workgroups < -1) return fail(LBM_EINVAL, "workgroups must be -1 (keep), 0 or positive");
   LBM_HIP_TRY(hipStreamSynchronize(b->stream));
   b->steps_per_launch = steps_per_launch;
   if (tile_x) b->tile2_x = tile_x;
   if (tile_rows) b->tile2_rows = tile_rows;
+  if (workgroups >= 0) b->workgroups2 = workgroups;
   b->drop_graphs();
   return b->build_tiles2();
 }
 
 int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, int* tile_x,
-                                   int* tile_rows) {
+                                   int* tile_rows, int* workgroups) {
   if (!b) return fail(LBM_EINVAL, "null batch handle");
   if (steps_per_launch) *steps_per_launch = b->two_step() ? 2 : 1;
   if (tile_x) *tile_x = b->tile2_x;
   if (tile_rows) *tile_rows = b->tile2_rows;
+  if (workgroups) *workgroups = b->workgroups2;
   return LBM_OK;
 }
 
@@ -575,32 +583,44 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
   // The two-step tiles, where run() takes them; best_mlups is then the rate of that path.
   if (b->steps_per_launch == 2) {
     double best2 = -1.0;
-    int b2x = b->tile2_x, b2r = b->tile2_rows;
+    int b2x = b->tile2_x, b2r = b->tile2_rows, b2w = b->workgroups2, b2nt = b->store_nt;
+    const int keep_nt = b->store_nt;
+    // One workgroup per tile (persistent grids of 2-4 per CU measured 8-25 % slower on the
+    // seeded domains; lbm_batch_set_steps_per_launch still selects them); at least 8 launches
+    // per candidate, the differences between the best shapes being a few percent.
+    const int pairs = std::max(8, (steps + 1) / 2);
+    for (int nt : {0, 1})
     for (int tx : {64, 128, 256, 512})
       for (int r : {16, 32, 64}) {
         if (step2_lds_bytes(tx, b->rs()) > 64 * 1024) continue;
         b->tile2_x = tx;
         b->tile2_rows = r;
+        b->workgroups2 = 0;
+        b->store_nt = nt;
         LBM_TRY(b->build_tiles2());
         int par = b->parity;
         LBM_TRY(b->issue(2, par));  // warm-up
         LBM_HIP_TRY(hipEventRecord(b->ev0, b->stream));
-        LBM_TRY(b->issue(2 * ((steps + 1) / 2), par));
+        LBM_TRY(b->issue(2 * pairs, par));
         LBM_HIP_TRY(hipEventRecord(b->ev1, b->stream));
         LBM_HIP_TRY(hipEventSynchronize(b->ev1));
         b->parity = par;
         float ms = 0.0f;
         LBM_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        const double mlups =
-            static_cast<double>(b->total_cells) * 2 * ((steps + 1) / 2) / (ms * 1e-3) / 1e6;
+        const double mlups = static_cast<double>(b->total_cells) * 2 * pairs / (ms * 1e-3) / 1e6;
         if (mlups > best2) {
           best2 = mlups;
           b2x = tx;
           b2r = r;
+          b2w = 0;
+          b2nt = nt;
         }
       }
     b->tile2_x = b2x;
     b->tile2_rows = b2r;
+    b->workgroups2 = b2w;
+    // the stores option is shared by both kernels: the two-step one's choice when it runs
+    b->store_nt = best2 > best ? b2nt : keep_nt;
     LBM_TRY(b->build_tiles2());
     if (best2 > 0.0) best = best2;
   }

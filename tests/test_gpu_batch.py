@@ -108,12 +108,13 @@ def test_two_step_batch_matches_one_step_batch(lbm, oracle_mod, prec):
         b = lbm.Batch(doms, prec, init=init)
         b.set_graph_steps(6)
         if spl == 2:
-            for tile, steps in (((64, 16), 6), ((256, 32), 5), ((512, 64), 7)):
+            # one workgroup per tile, and persistent grids smaller than the tile count
+            for tile, wg, steps in (((64, 16), 0, 6), ((256, 32), 7, 5), ((512, 64), 300, 7)):
                 if prec == "double":  # its LDS ring holds twice the bytes: at most 256 columns
                     tile = (min(tile[0], 256), tile[1])
-                b.set_steps_per_launch(2, *tile)
+                b.set_steps_per_launch(2, *tile, wg)
                 assert b.steps_per_launch() == {"steps_per_launch": 2, "tile_x": tile[0],
-                                                "tile_rows": tile[1]}
+                                                "tile_rows": tile[1], "workgroups": wg}
                 b.run(steps)
         else:
             b.set_steps_per_launch(1)
