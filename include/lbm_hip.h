@@ -86,26 +86,26 @@ int lbm_get_block(const lbm_solver* s, int* block_x, int* rows_per_thread);
  * rows per XCD (row r on XCD r % 8: the cache lines that neighbouring tiles of a row share stay
  * in one L2; needs a multiple of 8 tile rows, else round-robin). Defaults: fp32 1, 1, 0;
  * fp64 1, 1, 1; FP16 0, 1, 2. */
-int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource, int tile_order);
+int lbm_set_kernel_options(lbm_solver* s, int streaming_stores, int single_resource,
+                           int tile_order);
 int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* single_resource,
                            int* tile_order);
 
 /* No reference counterpart: temporal blocking. With steps_per_launch = 2 (the default)
  * lbm_run and lbm_timed_run advance pairs of steps in one launch of a two-step kernel wherever
- * it applies (periodic domain, uniform force; float, double or FP16 storage): every population is read from
- * HBM and written once per two steps. The results are bit-identical to single steps (same
- * operations per cell). block_x (64..512, multiple of 64; 0 keeps the current value) and rows
- * set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64, 48, 32, 24, 16
- * rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps the current
- * setting; tile_order (0, 1, 2 as in lbm_set_kernel_options, -1 keeps; default 1) orders its
- * tiles over the XCDs. lbm_step always advances one
- * step. lbm_get_steps_per_launch reports the height in use and in_use whether the current
- * configuration takes the two-step path. */
+ * it applies (periodic domain, uniform force; float, double or FP16 storage): every population
+ * is read from HBM and written once per two steps. The results are bit-identical to single
+ * steps (same operations per cell). block_x (64..512, multiple of 64; 0 keeps the current
+ * value) and rows set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64,
+ * 48, 32, 24, 16 rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps
+ * the current setting; tile_order (0, 1, 2 as in lbm_set_kernel_options, -1 keeps; default 1)
+ * orders its tiles over the XCDs. lbm_step always advances one step. lbm_get_steps_per_launch
+ * reports the height in use and in_use whether the current configuration takes the two-step
+ * path. */
 int lbm_set_steps_per_launch(lbm_solver* s, int steps_per_launch, int block_x, int rows,
                              int tile_order);
 int lbm_get_steps_per_launch(const lbm_solver* s, int* steps_per_launch, int* block_x, int* rows,
-                             int* tile_order,
-                             int* in_use);
+                             int* tile_order, int* in_use);
 
 /* SolverV3::step() — solver_v3.cu:251-278: one asynchronous step. */
 int lbm_step(lbm_solver* s);

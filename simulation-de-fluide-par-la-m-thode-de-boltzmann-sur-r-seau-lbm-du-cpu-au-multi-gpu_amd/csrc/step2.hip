@@ -1,14 +1,15 @@
-// Two fused time steps per launch (temporal blocking), periodic domain with a uniform force
-// (float, double, and FP16 storage with its own cell update and rounding):
+// Two fused time steps per launch (temporal blocking), periodic domain with a uniform force:
 // R_{t+2} from R_t with every population read once from HBM and written once, i.e. 72 B (fp32)
 // per two cell updates instead of per one. Each cell update performs exactly the operations of
-// step_kernel (the same pull, the same bounce-back rule, the same collide()), so the result is
-// bit-identical to two single steps (src/v3_cuda_opt/solver_v3.cu:45-113 applied twice).
+// the one-step kernel (the same pull, the same bounce-back rule, the same cell update: the
+// reference's collide() for float and double, step_fp16_kernel's collision and rounding for
+// FP16 storage), so the result is bit-identical to two single steps
+// (src/v3_cuda_opt/solver_v3.cu:45-113 applied twice).
 //
 // A workgroup owns a tile of W = blockDim.x columns by H rows of the output and walks its rows
 // downwards (in the +y direction). At row r it
-//   1. computes step 1 (R_t -> R_{t+1}) at row r for its W columns, pulling from HBM (rows r - 1,
-//      r, r + 1 of R_t; the row loads of neighbouring iterations hit L2);
+//   1. computes step 1 (R_t -> R_{t+1}) at row r for its W columns, pulling from HBM (plane i
+//      from row r - c_y,i of R_t: each plane row is read once per tile);
 //   2. publishes the six populations its neighbour columns pull in step 2 (planes with c_x != 0)
 //      in a four-row ring in LDS;
 //   3. computes step 2 (R_{t+1} -> R_{t+2}) at row r - 1: planes with c_x = 0 come from the
@@ -22,9 +23,9 @@
 // barrier per row: with four ring slots a slot is rewritten two barriers after its last read.
 // The recomputed rows and columns are the overhead of the scheme (rows y0 - 1 and y1 and their
 // sources y0 - 2 and y1 + 1 are read by two tiles); step2_tile_rows() picks tiles of up to 64
-// rows whose count fills the resident workgroup slots evenly. Solid cells are never written (both population
-// buffers keep the values every upload gives them); their results go to discard slots past
-// each plane's last cell.
+// rows whose count fills the resident workgroup slots evenly. Solid cells are never written
+// (both population buffers keep the values every upload gives them); their results go to
+// discard slots past each plane's last cell.
 #include <algorithm>
 #include <type_traits>
 
@@ -54,8 +55,8 @@ __device__ __forceinline__ Rows rows_of(int r, int nx, int ny, unsigned B) {
   return Rows{static_cast<unsigned>(rm * nx) * B, static_cast<unsigned>(r * nx) * B,
               static_cast<unsigned>(rp * nx) * B};
 }
-// The same for a row that is wave-uniform, stated so: the compiler treats values computed in
-// the (divergent) prologue loop as per-lane, and a per-lane scalar offset costs a loop per load.
+// The same for a row that is wave-uniform, stated so: where the compiler cannot prove an offset
+// uniform it keeps it per lane, and a per-lane scalar offset costs a loop per load.
 __device__ __forceinline__ Rows uniform_rows_of(int r, int nx, int ny, unsigned B) {
   const Rows a = rows_of(r, nx, ny, B);
   auto u = [](unsigned x) { return static_cast<unsigned>(__builtin_amdgcn_readfirstlane(x)); };
