@@ -442,3 +442,31 @@ def test_fp16_two_step_equals_one_step_launches(lbm, grid, tile):
         s.close()
     for k, (a, b) in enumerate(zip(*out)):
         assert_bitwise(a, b, f"FP16 {grid} tile {tile} after run {k}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("grid", [(1, 1), (1, 7), (5, 1), (3, 3), (64, 2), (513, 20), (130, 5)])
+def test_two_step_degenerate_grids_match_oracle(lbm, oracle_mod, prec, grid):
+    """Fully periodic domains (no walls, an obstacle cell where there is room) down to one column
+    or one row, and a last column tile one cell wide (513 = 512 + 1): the two-step path against
+    the oracle, bit for bit, after odd and even step counts."""
+    nx, ny = grid
+    solid = np.zeros(nx * ny, np.uint8)
+    if nx >= 3 and ny >= 3:
+        solid[(ny // 2) * nx + nx // 2] = 1
+    rng = np.random.default_rng(nx * 131 + ny)
+    dt = np.float32 if prec == "float" else np.float64
+    S = (rng.standard_normal(nx * ny * 9) * 1e-3).astype(dt)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    s = lbm.Solver(nx, ny, 0.9, 1e-5, -2e-6, solid, prec)
+    assert s.steps_per_launch()["in_use"]
+    o = oracle_mod.Oracle(nx, ny, 0.9, 1e-5, -2e-6, solid, prec, "v0")
+    for x in (s, o):
+        x.set_populations(S)
+    done = 0
+    for n in (2, 3, 4):
+        s.run(n)
+        o.run(n)
+        done += n
+        assert_bitwise(s.populations(), o.populations(), f"{grid} after {done} steps")
+    s.close()
