@@ -118,6 +118,12 @@ int lbm_timed_run(lbm_solver* s, int steps, double* seconds);
 /* SolverV3::macroscopic — solver_v3.cu:322-343: rho, ux, uy (N reals each) from the
  * post-streaming state, computed on the device with v0's operations; solid cells get 0. */
 int lbm_macroscopic(lbm_solver* s, void* rho, void* ux, void* uy);
+/* lbm::vorticity(ux, uy, solid, nx, ny, periodic_x) — src/common/io.cpp:54-74, applied to the
+ * solver's macroscopic velocity: N reals (float for FP16 storage), 0 in solid cells, computed
+ * on the device with the reference's operations. periodic_x: 1 periodic, 0 one-sided
+ * differences at the open ends (the cylinder driver's choice, driver.hpp:229,245), -1 the
+ * solver's own x boundary. */
+int lbm_vorticity(lbm_solver* s, void* out, int periodic_x);
 /* SolverV3::total_mass — solver_v3.cu:345-355: fluid cells + sum of g, FP64 on the device
  * (deterministic tree order, not v0's sequential order). */
 int lbm_total_mass(lbm_solver* s, double* mass);

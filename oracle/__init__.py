@@ -12,7 +12,9 @@ Function by function, what pins it (besides the reference's own outputs in tests
   * ``PushAoS`` step — golden fixtures of reference v0 (float/double, walls + block,
     cylinder with open x), Poiseuille exactness at tau = 1/2 + sqrt(3)/4, mass conservation;
   * ``PullSoA`` step — bitwise equal to ``PushAoS`` for 1, 2, 3, 4, 8 threads;
-  * ``pull_to_aos`` / ``aos_to_pull`` — inverse of each other on fluid slots (round trip).
+  * ``pull_to_aos`` / ``aos_to_pull`` — inverse of each other on fluid slots (round trip);
+  * ``vorticity`` — bitwise equal to the reference's io.cpp vorticity (oracle/_ref) on random
+    fields, periodic and open; exact on linear shear fields (interior and one-sided columns).
 """
 from __future__ import annotations
 
@@ -230,6 +232,26 @@ def guo(wi, cxi, cyi, ux, uy, fx, fy):
     return lib().oracle_guo_f64(wi, cxi, cyi, ux, uy, fx, fy)
 
 
+def vorticity(ux, uy, solid, nx, ny, periodic_x=True):
+    """dUy/dx - dUx/dy as src/common/io.cpp:54-74 computes it, in the dtype of ux: central
+    differences over (x +- 1) / 2 and (y +- 1) / 2 with periodic wrap; with periodic_x False the
+    columns x = 0 and x = nx - 1 take one-sided differences over h = 1; 0 in solid cells."""
+    t = np.asarray(ux).dtype.type
+    u = np.asarray(ux).reshape(ny, nx)
+    v = np.asarray(uy).reshape(ny, nx)
+    xp = np.roll(np.arange(nx), -1)
+    xm = np.roll(np.arange(nx), 1)
+    h = np.full(nx, 2, dtype=t)
+    if not periodic_x:
+        xm[0], h[0] = 0, 1
+        xp[nx - 1], h[nx - 1] = nx - 1, 1
+    duy_dx = (v[:, xp] - v[:, xm]) / h[None, :]
+    dux_dy = (np.roll(u, -1, axis=0) - np.roll(u, 1, axis=0)) / t(2)
+    w = (duy_dx - dux_dy).astype(t)
+    w[np.asarray(solid).reshape(ny, nx) != 0] = t(0)
+    return w.reshape(-1)
+
+
 # ---- the reference's own CPU solvers (oracle/_ref), when they were built here ----------------
 
 class Reference:
@@ -250,6 +272,8 @@ class Reference:
         L.ref_v1_channel_timed.argtypes = [i, i, d, d, i, i, C.POINTER(i)]
         L.ref_v1_channel_timed.restype = d
         L.ref_real_size.restype = i
+        L.ref_vorticity.argtypes = [vp, vp, _u8p, i, i, i, vp]
+        L.ref_write_npy.argtypes = [C.c_char_p, vp, i, i, i]
         self.L = L
         self.dtype = np.float32 if L.ref_real_size() == 4 else np.float64
 
@@ -266,6 +290,23 @@ class Reference:
           int(open_x), float(u_in), float(init[0]), float(init[1]), float(init[2]), int(steps),
           out.ctypes.data)
         return out
+
+    def vorticity(self, ux, uy, solid, nx, ny, periodic_x=True):
+        u = np.ascontiguousarray(ux, dtype=self.dtype).reshape(-1)
+        v = np.ascontiguousarray(uy, dtype=self.dtype).reshape(-1)
+        out = np.empty(nx * ny, dtype=self.dtype)
+        self.L.ref_vorticity(u.ctypes.data, v.ctypes.data,
+                             np.ascontiguousarray(solid, np.uint8).reshape(-1), nx, ny,
+                             int(bool(periodic_x)), out.ctypes.data)
+        return out
+
+    def write_npy(self, path, a, ny, nx):
+        """The reference's writer: the solver real, or uint8 for a uint8 array."""
+        a = np.ascontiguousarray(a)
+        kind = 1 if a.dtype == np.uint8 else 0
+        if kind == 0:
+            a = np.ascontiguousarray(a, dtype=self.dtype)
+        self.L.ref_write_npy(str(path).encode(), a.ctypes.data, int(ny), int(nx), kind)
 
     def v1_channel_timed(self, nx, ny, tau, fx, warmup, steps):
         t = C.c_int()

@@ -1,4 +1,5 @@
-// ORACLE — test infrastructure only. A C-ABI shim over the reference's own CPU solvers (v0, v1),
+// ORACLE — test infrastructure only. A C-ABI shim over the reference's own CPU solvers (v0, v1)
+// and its field writers (io.cpp: write_npy, vorticity),
 // compiled from the sources where they lie under the reference checkout by oracle/Makefile
 // (targets ref-float / ref-double) into oracle/_ref/. No reference source is copied here: this
 // file only includes the reference headers and forwards to SolverV0 / SolverV1.
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "geometry.hpp"
+#include "io.hpp"
 #include "real.hpp"
 #include "solver_v0.hpp"
 #include "solver_v1.hpp"
@@ -81,6 +83,30 @@ double ref_v1_channel_timed(int nx, int ny, double tau, double fx, int warmup, i
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (threads) *threads = s.threads();
   return t;
+}
+
+// io.cpp's vorticity (src/common/io.cpp:54-74) of caller-owned ux, uy (nx * ny reals).
+void ref_vorticity(const void* ux, const void* uy, const std::uint8_t* solid, int nx, int ny,
+                   int periodic_x, void* out) {
+  const std::size_t n = static_cast<std::size_t>(nx) * ny;
+  const real* u = static_cast<const real*>(ux);
+  const real* v = static_cast<const real*>(uy);
+  const std::vector<real> a(u, u + n), b(v, v + n);
+  const lbm::Mask m(solid, solid + n);
+  const std::vector<real> w = lbm::vorticity(a, b, m, nx, ny, periodic_x != 0);
+  std::memcpy(out, w.data(), n * sizeof(real));
+}
+
+// io.cpp's write_npy (src/common/io.cpp:11-52): kind 0 the solver real, 1 uint8.
+void ref_write_npy(const char* path, const void* a, int ny, int nx, int kind) {
+  const std::size_t n = static_cast<std::size_t>(nx) * ny;
+  if (kind == 1) {
+    const std::uint8_t* p = static_cast<const std::uint8_t*>(a);
+    lbm::write_npy(path, std::vector<std::uint8_t>(p, p + n), ny, nx);
+  } else {
+    const real* p = static_cast<const real*>(a);
+    lbm::write_npy(path, std::vector<real>(p, p + n), ny, nx);
+  }
 }
 
 }  // extern "C"

@@ -80,6 +80,7 @@ def _declare(L):
         "lbm_run": ([vp, i], i),
         "lbm_timed_run": ([vp, i, dp], i),
         "lbm_macroscopic": ([vp, vp, vp, vp], i),
+        "lbm_vorticity": ([vp, vp, i], i),
         "lbm_total_mass": ([vp, dp], i),
         "lbm_momentum": ([vp, dp, dp], i),
         "lbm_populations": ([vp, vp], i),
@@ -308,6 +309,13 @@ class Solver:
         r, u, v = (np.empty(self.N, dtype=self.dtype) for _ in range(3))
         check(lib().lbm_macroscopic(self._h, r.ctypes.data, u.ctypes.data, v.ctypes.data))
         return r, u, v
+
+    def vorticity(self, periodic_x: int = -1) -> np.ndarray:
+        """io.cpp:54-74 vorticity of the current velocity, on the device: periodic_x 1
+        periodic, 0 one-sided at the open ends, -1 the solver's own x boundary."""
+        w = np.empty(self.N, dtype=self.dtype)
+        check(lib().lbm_vorticity(self._h, w.ctypes.data, int(periodic_x)))
+        return w
 
     def total_mass(self) -> float:
         m = C.c_double()

@@ -41,7 +41,52 @@ __global__ void __launch_bounds__(kReduceThreads)
   }
 }
 
+// Vorticity of a velocity field (src/common/io.cpp:54-74): central differences, periodic in y;
+// in x periodic, or one-sided (step 1) at x = 0 and x = nx - 1 of an open channel; 0 in solid
+// cells. The same operations in T as the reference's host loop.
+template <class T>
+__global__ void vorticity_kernel(const T* __restrict__ ux, const T* __restrict__ uy,
+                                 const uint16_t* __restrict__ links, int nx, int ny,
+                                 int periodic_x, T* __restrict__ w) {
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= static_cast<size_t>(nx) * ny) return;
+  if (links[c] & kLinkSolid) {
+    w[c] = T(0);
+    return;
+  }
+  const int x = static_cast<int>(c % nx), y = static_cast<int>(c / nx);
+  const int yp = y == ny - 1 ? 0 : y + 1, ym = y == 0 ? ny - 1 : y - 1;
+  int xp = x == nx - 1 ? 0 : x + 1, xm = x == 0 ? nx - 1 : x - 1;
+  T h = T(2);
+  if (!periodic_x && x == 0) {
+    xm = 0;
+    h = T(1);
+  }
+  if (!periodic_x && x == nx - 1) {
+    xp = nx - 1;
+    h = T(1);
+  }
+  const size_t row = static_cast<size_t>(y) * nx;
+  const T duy_dx = (uy[row + xp] - uy[row + xm]) / h;
+  const T dux_dy = (ux[static_cast<size_t>(yp) * nx + x] - ux[static_cast<size_t>(ym) * nx + x]) / T(2);
+  w[c] = duy_dx - dux_dy;
+}
+
 }  // namespace
+
+template <class T>
+hipError_t launch_vorticity(const T* ux, const T* uy, const uint16_t* links, int nx, int ny,
+                            int periodic_x, T* w, hipStream_t stream) {
+  const size_t cells = static_cast<size_t>(nx) * ny;
+  if (cells == 0) return hipSuccess;
+  vorticity_kernel<T><<<static_cast<unsigned>((cells + 255) / 256), 256, 0, stream>>>(
+      ux, uy, links, nx, ny, periodic_x, w);
+  return hipGetLastError();
+}
+template hipError_t launch_vorticity<float>(const float*, const float*, const uint16_t*, int, int,
+                                            int, float*, hipStream_t);
+template hipError_t launch_vorticity<double>(const double*, const double*, const uint16_t*, int,
+                                             int, int, double*, hipStream_t);
 
 template <class S>
 hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,

@@ -147,6 +147,12 @@ hipError_t launch_reduce(const GatherParams<T>& p, double* partials, double* out
 hipError_t launch_reduce_fp16(const GatherParamsH& p, double* partials, double* out,
                               hipStream_t stream);
 
+// Vorticity of the velocity field ux, uy (device arrays of nx * ny) into w, 0 in solid cells
+// (diag.hip; src/common/io.cpp:54-74); periodic_x = 0: one-sided differences at the open ends.
+template <class T>
+hipError_t launch_vorticity(const T* ux, const T* uy, const uint16_t* links, int nx, int ny,
+                            int periodic_x, T* w, hipStream_t stream);
+
 // Momentum-exchange force (FP64, out[0..1]) on the solids next to the fluid cells of the box
 // [x0, x0 + w) x [y0, y0 + h) (diag.hip). S = float, double or uint16_t (FP16 storage).
 template <class S>

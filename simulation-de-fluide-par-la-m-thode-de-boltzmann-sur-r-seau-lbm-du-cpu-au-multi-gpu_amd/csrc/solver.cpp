@@ -178,6 +178,11 @@ int lbm_macroscopic(lbm_solver* s, void* rho, void* ux, void* uy) {
   return s->macroscopic(rho, ux, uy);
 }
 
+int lbm_vorticity(lbm_solver* s, void* out, int periodic_x) {
+  LBM_TRY(check_handle(s));
+  return s->vorticity(out, periodic_x);
+}
+
 int lbm_total_mass(lbm_solver* s, double* mass) {
   LBM_TRY(check_handle(s));
   double r[4];

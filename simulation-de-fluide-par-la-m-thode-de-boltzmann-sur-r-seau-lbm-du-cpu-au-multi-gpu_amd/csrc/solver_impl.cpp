@@ -477,6 +477,35 @@ int SolverState::macroscopic(void* rho, void* ux, void* uy) {
   return LBM_OK;
 }
 
+int SolverState::vorticity(void* out, int periodic_x) {
+  if (!out) return fail(LBM_EINVAL, "null output array");
+  const size_t rs = real_size(prec), n = cells();
+  LBM_TRY(ensure_staging(4 * n * rs));
+  char* st = static_cast<char*>(staging);
+  hipError_t e;
+  const int px = periodic_x < 0 ? (open_x ? 0 : 1) : (periodic_x ? 1 : 0);
+  if (prec == LBM_DOUBLE) {
+    double* f = reinterpret_cast<double*>(st);
+    e = launch_macro(make_gather_params<double>(*this), f, f + n, f + 2 * n, stream);
+    if (e == hipSuccess)
+      e = launch_vorticity(f + n, f + 2 * n, links, nx, ny, px, f + 3 * n, stream);
+  } else {
+    float* f = reinterpret_cast<float*>(st);
+    if (prec == LBM_FLOAT)
+      e = launch_macro(make_gather_params<float>(*this), f, f + n, f + 2 * n, stream);
+    else
+      e = launch_macro_fp16(make_gather_params_h(*this), static_cast<float>(fx),
+                            static_cast<float>(fy), static_cast<const float*>(force), f, f + n,
+                            f + 2 * n, stream);
+    if (e == hipSuccess)
+      e = launch_vorticity(f + n, f + 2 * n, links, nx, ny, px, f + 3 * n, stream);
+  }
+  if (e != hipSuccess) return fail(LBM_EHIP, std::string("vorticity launch: ") + hipGetErrorString(e));
+  LBM_HIP_TRY(hipStreamSynchronize(stream));
+  LBM_HIP_TRY(hipMemcpy(out, st + 3 * n * rs, n * rs, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
 int SolverState::reduce(double (&out)[4]) {
   hipError_t e;
   double* partials = red;

@@ -68,6 +68,7 @@ struct SolverState {
   int advance(int n);
   bool two_step_eligible() const;
   int two_step_rows() const;  // tile rows the two-step kernel uses
+  int vorticity(void* out, int periodic_x);  // periodic_x < 0: the solver's own x boundary
   int populations(void* out);
   int populations_at(int x, int y, void* out9);
   int set_populations(const void* in);
