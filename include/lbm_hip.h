@@ -143,7 +143,11 @@ int lbm_set_state(lbm_solver* s, const void* in);
 
 /* SolverV3::threads — 0 on the GPU. */
 int lbm_threads(const lbm_solver* s);
-/* SolverV3::bench_info — solver_v3.cu:357-370: extra JSON fields (no braces). */
+/* SolverV3::bench_info — solver_v3.cu:357-370: extra JSON fields (no braces) for the BENCH line:
+ * gpu (the device name, or "AMD <arch>" where the runtime reports none), arch, hip_runtime,
+ * hip_driver, block (one-step launch shape), variant ("soa_pull_two_step" when run() advances
+ * two steps per launch, else "soa_pull_buffer"), storage, stores, planes, tile_order,
+ * plane_stride, steps_per_launch, two_step_tile, two_step_order. */
 int lbm_bench_info(const lbm_solver* s, char* buf, size_t len);
 int lbm_dims(const lbm_solver* s, int* nx, int* ny, int* precision);
 

@@ -275,18 +275,26 @@ int lbm_bench_info(const lbm_solver* s, char* buf, size_t len) {
   LBM_HIP_TRY(hipRuntimeGetVersion(&rt));
   LBM_HIP_TRY(hipDriverGetVersion(&drv));
   static const char* names[] = {"float", "double", "fp16-storage"};
+  // Some runtimes report an empty marketing name: fall back to the architecture.
+  char arch[64] = {0};
+  std::snprintf(arch, sizeof arch, "%s", prop.gcnArchName);
+  if (char* colon = std::strchr(arch, ':')) *colon = 0;
+  char gpu[300];
+  std::snprintf(gpu, sizeof gpu, prop.name[0] ? "%s" : "AMD %s", prop.name[0] ? prop.name : arch);
+  const bool two = s->two_step_eligible();  // run() advances pairs of steps per launch
   if (buf && len)
     std::snprintf(buf, len,
                   "\"gpu\": \"%s\", \"arch\": \"%s\", \"hip_runtime\": %d, \"hip_driver\": %d, "
-                  "\"block\": \"%dx%d\", \"variant\": \"soa_pull_buffer\", \"storage\": \"%s\", "
+                  "\"block\": \"%dx%d\", \"variant\": \"%s\", \"storage\": \"%s\", "
                   "\"stores\": \"%s\", \"planes\": \"%s\", \"tile_order\": \"%s\", "
                   "\"plane_stride\": %zu, \"steps_per_launch\": %d, \"two_step_tile\": \"%dx%d\", "
                   "\"two_step_order\": %d",
-                  prop.name, prop.gcnArchName, rt, drv, s->block_x, s->rows, names[s->prec],
+                  gpu, prop.gcnArchName, rt, drv, s->block_x, s->rows,
+                  two ? "soa_pull_two_step" : "soa_pull_buffer", names[s->prec],
                   s->store_nt ? "nontemporal" : "default",
                   s->one_rsrc ? "one_resource" : "resource_per_plane",
                   s->xcd_bands == 1 ? "xcd_bands" : (s->xcd_bands == 2 ? "xcd_rows" : "round_robin"),
-                  s->S, s->two_step_eligible() ? 2 : 1, s->block2_x, s->two_step_rows(),
+                  s->S, two ? 2 : 1, s->block2_x, s->two_step_rows(),
                   s->two_step_order);
   return LBM_OK;
 }

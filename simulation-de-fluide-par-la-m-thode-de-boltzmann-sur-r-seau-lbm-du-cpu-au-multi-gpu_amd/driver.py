@@ -118,12 +118,13 @@ def _compiler(info: dict) -> str:
     return f"hipcc (HIP {v // 10000000}.{v // 100000 % 100}.{v % 100000})"
 
 
-def run_bench(lbm, solver, p: Params, version: str, out=sys.stdout) -> None:
+def run_bench(lbm, solver, p: Params, version: str, out=None) -> None:
     """driver.hpp:115-132."""
     import json
 
     from lbm_amd import formats
 
+    out = out or sys.stdout
     solver.run(p.warmup)
     frag = solver.bench_info()
     info = json.loads("{" + frag + "}")
