@@ -42,16 +42,23 @@ __host__ __device__ constexpr uint32_t outlet_bb_bit(int i) {
 // Every product by a lattice constant is written as in the reference (int constant converted
 // to T); the compiler may only apply IEEE-exact folds (1*x, -1*x, x + -y), never 0*x.
 //
-// Two terms of the reference's expressions are left out because they cannot change a bit of
-// any result that is finite (a non-finite state has no bitwise counterpart: NaN encodings
-// differ between processors):
-//  * the products 0 * g_i in the sums over c_x g_i and c_y g_i: each running sum starts at +0,
-//    and a round-to-nearest sum is -0 only when both operands are -0, so the sum is never -0
-//    and adding +-0 to it is the identity;
-//  * 3 cu + 4.5 cu cu of the rest direction: there cu = 0 * ux + 0 * uy = +-0, so 3 cu is +-0,
-//    4.5 cu * cu is +0 and their sum is +0 whatever the signs; the bracket is +0 - 1.5 usq.
-// Both are checked bit for bit against the reference's full expressions on random and
-// signed-zero / subnormal inputs (tests/cpp/test_collide.cpp).
+// Three rewrites of the reference's expressions cannot change a bit of any result that is
+// finite (a non-finite state has no bitwise counterpart: NaN encodings differ between
+// processors):
+//  * the products 0 * g_i in the sums over c_x g_i and c_y g_i are left out: each running sum
+//    starts at +0, and a round-to-nearest sum is -0 only when both operands are -0, so the sum
+//    is never -0 and adding +-0 to it is the identity;
+//  * 3 cu + 4.5 cu cu of the rest direction is left out: there cu = 0 * ux + 0 * uy = +-0, so
+//    3 cu is +-0, 4.5 cu * cu is +0 and their sum is +0 whatever the signs; the bracket is
+//    +0 - 1.5 usq;
+//  * directions 3, 4, 7, 8 reuse the bracket terms of their opposites 1, 2, 5, 6: cu of a
+//    direction and of its opposite are exact negatives unless both are zeros (round to
+//    nearest is symmetric; the zero products only add +-0), so 4.5 cu * cu is the same value
+//    and 3 cu the exact negative, making 3 cu + 4.5 cu cu = 4.5 cu cu - 3 cu of the opposite;
+//    where cu is a zero both sums are +0, as above.
+// Everything else, cu and cf in the source term included, keeps the reference's operations.
+// All three are checked bit for bit against the reference's full expressions on random,
+// signed-zero and subnormal inputs (tests/cpp/test_collide.cpp).
 template <class T>
 __host__ __device__ __forceinline__ void collide(T (&g)[kQ], T fx, T fy, T omega, T fp) {
   T m0 = T(0), mx = T(0), my = T(0);
@@ -66,14 +73,27 @@ __host__ __device__ __forceinline__ void collide(T (&g)[kQ], T fx, T fy, T omega
   const T ux = (mx + T(0.5) * fx) / rho;
   const T uy = (my + T(0.5) * fy) / rho;
   const T usq = ux * ux + uy * uy;
+  T cu[kQ], t3[kQ], t45[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) cu[i] = T(kCx[i]) * ux + T(kCy[i]) * uy;
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    if (i == 1 || i == 2 || i == 5 || i == 6) {
+      t3[i] = T(3) * cu[i];
+      t45[i] = T(4.5) * cu[i] * cu[i];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
     const T cxi = T(kCx[i]), cyi = T(kCy[i]), wi = weight<T>(i);
-    const T cu = cxi * ux + cyi * uy;
-    const T bracket = i == 0 ? T(0) - T(1.5) * usq : T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq;
+    const int o = kOpp[i];
+    const bool first = i == 1 || i == 2 || i == 5 || i == 6;
+    const T bracket = i == 0 ? T(0) - T(1.5) * usq
+                      : first ? t3[i] + t45[i] - T(1.5) * usq
+                              : t45[o] - t3[o] - T(1.5) * usq;
     const T geq = wi * (drho + rho * bracket);
     const T cf = cxi * fx + cyi * fy;
-    const T src = wi * (T(3) * ((cxi - ux) * fx + (cyi - uy) * fy) + T(9) * cu * cf);
+    const T src = wi * (T(3) * ((cxi - ux) * fx + (cyi - uy) * fy) + T(9) * cu[i] * cf);
     g[i] = g[i] - omega * (g[i] - geq) + fp * src;
   }
 }

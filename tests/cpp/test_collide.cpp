@@ -2,8 +2,10 @@
 // full (src/v0_cpu/solver_v0.cpp:26-39,72-83 with src/common/d2q9.hpp:40-50: every product by
 // a lattice constant, zeros included, in the reference's order), bit for bit, in float and
 // double. Inputs: f - w populations at physical scale, signed zeros, subnormals, mixed
-// magnitudes; forces of both signs, +0 and -0. Cases whose reference result is not finite are
-// counted and skipped (no bitwise counterpart). Host code only: no GPU is used.
+// magnitudes; forces of both signs, +0 and -0; tau over (0.505, 2.005), and in a second run
+// also the largest tau below 2 and populations with a -0 next to a subnormal (where an
+// equilibrium can underflow to -0). Cases whose reference result is not finite are counted and skipped (no bitwise
+// counterpart). Host code only: no GPU is used.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -75,9 +77,17 @@ bool same_bits(T a, T b) {
   return std::memcmp(&a, &b, sizeof(T)) == 0;
 }
 
+// Largest tau below 2 whose omega = 1/tau is still above 1/2 in T.
 template <class T>
+T tau_below_two() {
+  T t = std::nextafter(T(2), T(0));
+  while (!(T(1) / t > T(0.5))) t = std::nextafter(t, T(0));
+  return t;
+}
+
+template <class T, bool kEdge>
 int run(const char* name, long cases) {
-  Rng r{0x9E3779B97F4A7C15ull};
+  Rng r{kEdge ? 0x2545F4914F6CDD1Dull : 0x9E3779B97F4A7C15ull};
   long compared = 0, skipped = 0, bad = 0;
   for (long n = 0; n < cases; ++n) {
     const int mode = static_cast<int>(n % 7);  // 6: every population of its own kind
@@ -94,9 +104,24 @@ int run(const char* name, long cases) {
                                                              : static_cast<T>(r.normal() * 1e-4);
     };
     const T fx = force(fk), fy = force(static_cast<int>(r.next() % 5));
-    const T tau = static_cast<T>(0.505 + 1.5 * r.uniform());
+    T tau = static_cast<T>(0.505 + 1.5 * r.uniform());
+    if (kEdge) {
+      // 1 in 8: tau just below 2; 1 in 16 more: one population -0 with a subnormal neighbour
+      const uint64_t pick = r.next() % 16;
+      if (pick < 2) tau = tau_below_two<T>();
+      else if (tau >= T(2)) tau = static_cast<T>(1.0 + r.uniform());
+      if (pick == 2) {
+        const int i = static_cast<int>(r.next() % kQ);
+        g[i] = h[i] = -T(0);
+        g[(i + 1) % kQ] = h[(i + 1) % kQ] = std::numeric_limits<T>::denorm_min();
+      }
+    }
     const T omega = T(1) / tau;
     const T fp = T(1) - T(0.5) * omega;
+    if (kEdge && !(omega > T(0.5))) {
+      ++skipped;
+      continue;
+    }
     reference_collide(g, fx, fy, omega, fp);
     lbmhip::collide(h, fx, fy, omega, fp);
     bool finite = true;
@@ -123,8 +148,10 @@ int run(const char* name, long cases) {
 }  // namespace
 
 int main() {
-  int rc = run<float>("float", 2000000);
-  rc |= run<double>("double", 2000000);
+  int rc = run<float, false>("float collide", 2000000);
+  rc |= run<double, false>("double collide", 2000000);
+  rc |= run<float, true>("float collide, tau near 2 and -0 / subnormal pairs", 2000000);
+  rc |= run<double, true>("double collide, tau near 2 and -0 / subnormal pairs", 2000000);
   std::printf(rc == 0 ? "ALL PASSED\n" : "FAILED\n");
   return rc;
 }
