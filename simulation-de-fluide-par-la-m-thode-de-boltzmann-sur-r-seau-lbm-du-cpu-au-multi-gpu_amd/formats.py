@@ -11,7 +11,8 @@ so that the reference's analysis and plotting scripts read MI355X runs unchanged
   per timed repetition after an untimed warm-up, the CSV row run_bench.py aggregates them into
   (same columns: the CUDA-specific ones stay empty, the HIP versions go to gpu_driver), the copy
   ceiling CSV of bench/gpu_bandwidth.cu:92-96, and bench/ceiling.py's MLUPS ceiling.
-* Strouhal report (scripts/strouhal.py:84-100): the strouhal.csv row.
+* Strouhal report (scripts/strouhal.py:57-100): the analysis of a cylinder run's probe.csv and
+  the strouhal.csv row.
 """
 from __future__ import annotations
 
@@ -275,6 +276,36 @@ def ceiling_mlups(rows, real_size: int, peak_gbs: float | None = None, gpu: bool
     if peak_gbs:
         hw = peak_gbs * 1e9 / ((2 if gpu else 3) * Q * real_size) / 1e6
     return measured, hw
+
+
+def strouhal_analysis(out_dir, start: float = 0.5) -> dict:
+    """The Strouhal analysis of a cylinder run's output directory (scripts/strouhal.py:57-81):
+    probe.csv and meta.json read back as written, samples with step >= start * steps kept and
+    their mean removed; St from the Hann-windowed, 16x zero-padded FFT peak refined by a
+    parabola on the log spectrum, and from the mean period between upward zero crossings
+    (linear interpolation); both times D / u_in. Returns st_fft, st_zero, periods,
+    uy_amplitude (max |u_y - mean| over the window), window_start and the meta dict."""
+    out = pathlib.Path(out_dir)
+    meta = json.loads((out / "meta.json").read_text())
+    data = np.genfromtxt(out / "probe.csv", delimiter=",", names=True)
+    step, uy = data["step"], data["uy"]
+    keep = step >= start * meta["steps"]
+    step, uy = step[keep], uy[keep] - uy[keep].mean()
+    dt = float(step[1] - step[0])
+    D, U = meta["diameter"], meta["u_in"]
+    n = len(uy)
+    spec = np.abs(np.fft.rfft(uy * np.hanning(n), 16 * n))
+    freqs = np.fft.rfftfreq(16 * n, d=dt)
+    k = int(np.argmax(spec[1:]) + 1)
+    a, b, c = np.log(spec[k - 1:k + 2] + 1e-300)
+    f_fft = (k + 0.5 * (a - c) / (a - 2 * b + c)) * (freqs[1] - freqs[0])
+    up = np.where((uy[:-1] < 0) & (uy[1:] >= 0))[0]
+    t_cross = step[up] - uy[up] * (step[up + 1] - step[up]) / (uy[up + 1] - uy[up])
+    periods = np.diff(t_cross)
+    f_zero = 1.0 / periods.mean() if len(periods) else float("nan")
+    return {"st_fft": f_fft * D / U, "st_zero": f_zero * D / U, "periods": int(len(periods)),
+            "uy_amplitude": float(np.abs(uy).max()), "window_start": int(start * meta["steps"]),
+            "meta": meta}
 
 
 def strouhal_row(run, version, real, nx, ny, diameter, u_in, re, tau, steps, window_start,

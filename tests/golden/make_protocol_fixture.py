@@ -7,7 +7,8 @@ and README; no reference code is run).
   * the copy-ceiling CSV header and rows of results/gpu_bandwidth_Tesla-T4.csv and the STREAM
     rows of results/bandwidth_i5-10310U.csv;
   * the ceilings README.md:41-50 quotes for them (3,221 MLUPS GPU, 199 MLUPS CPU, float);
-  * the results/strouhal.csv header.
+  * the results/strouhal.csv header and rows (the reference's cylinder runs: 1000 x 320,
+    D 40, Re 100, u_in 0.05, 160,000 steps, St from the probe over the last 45 %).
 
 Usage (development container, reference checkout at /root/reference):
     python tests/golden/make_protocol_fixture.py
@@ -36,7 +37,7 @@ def main():
     h_gpu, r_gpu = rows("v3_blocks_Tesla-T4.csv", keep)
     h_bw, r_bw = rows("gpu_bandwidth_Tesla-T4.csv")
     _, r_stream = rows("bandwidth_i5-10310U.csv", ["threads", "copy_MBps"])
-    h_st, _ = rows("strouhal.csv")
+    h_st, r_st = rows("strouhal.csv")
     fixture = {
         "source": "reference results/*.csv and README.md:41-50",
         "run_bench_header_cpu": h_cpu, "run_bench_rows_cpu": r_cpu,
@@ -44,7 +45,7 @@ def main():
         "gpu_bandwidth_header": h_bw, "gpu_bandwidth_rows": r_bw,
         "stream_rows": r_stream,
         "readme_ceiling_mlups": {"gpu_float": 3221, "cpu_float": 199},
-        "strouhal_header": h_st,
+        "strouhal_header": h_st, "strouhal_rows": r_st,
     }
     (HERE / "bench_protocol.json").write_text(json.dumps(fixture, indent=1) + "\n")
     print(f"{len(r_cpu)} + {len(r_gpu)} run_bench rows, {len(r_bw)} bandwidth rows")

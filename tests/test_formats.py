@@ -317,3 +317,26 @@ def test_driver_cylinder_setup(driver):
     # results/strouhal.csv records tau 0.56000 for this case (Re 100, D 40, u_in 0.05)
     assert f"{cs['tau']:.5f}" == "0.56000"
     assert cs["tau"] == float(np.float32(0.5 + 3.0 * float(np.float32(0.05)) * 40 / 100))
+
+
+def test_strouhal_analysis_of_a_written_probe(formats, lbm, tmp_path):
+    # a probe.csv / meta.json pair as the cylinder driver writes them, u_y a sine of known
+    # period plus a decaying transient: the analysis finds St = D / (U T), agrees with the
+    # package's own estimator, and counts the periods in the window
+    D, U, T, every, steps = 40, 0.05, 4400.0, 50, 160000
+    n = np.arange(every, steps + 1, every)
+    uy = 0.03 * np.sin(2 * np.pi * n / T) + 0.01 * np.exp(-n / 5000.0)
+    with open(tmp_path / "probe.csv", "w") as f:
+        f.write("step,ux,uy,rho\n")
+        for k, v in zip(n, uy):
+            f.write(formats.probe_line(int(k), 0.05, float(v), 1.0))
+    meta = formats.cylinder_meta(1000, 320, 0.56, U, 100.0, D, 200.0, 161.0, 320, 161, every,
+                                 0, steps, "float")
+    (tmp_path / "meta.json").write_text(formats.meta_text(meta))
+    a = formats.strouhal_analysis(tmp_path, start=0.55)
+    assert a["window_start"] == 88000
+    assert abs(a["st_fft"] - D / (U * T)) < 2e-4 and abs(a["st_zero"] - D / (U * T)) < 2e-4
+    keep = n >= 88000
+    st_fft, st_zero, periods = lbm.strouhal(uy[keep], float(every), D, U)
+    assert abs(st_fft - a["st_fft"]) < 1e-4 and periods == a["periods"]
+    assert abs(a["uy_amplitude"] - 0.03) < 1e-3

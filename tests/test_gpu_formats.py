@@ -10,11 +10,14 @@
   - --bench: `reps` BENCH lines that run_bench.py's parser reads, with the reference's keys
     and consistent MLUPS, folded into a run_bench CSV row;
   - cylinder: probe.csv equals the oracle's probe (same sums, same text) line by line, the
-    vorticity frames and final fields equal the oracle's.
+    vorticity frames and final fields equal the oracle's;
+  - the reference's recorded cylinder case (results/strouhal.csv): St, periods and u_y
+    amplitude reproduced to the recorded digits.
 """
 from __future__ import annotations
 
 import json
+import pathlib
 
 import numpy as np
 import pytest
@@ -174,3 +177,35 @@ def test_driver_cylinder_probe_and_frames_equal_oracle(lbm, oracle_mod, driver, 
     assert (meta["case"], meta["diameter"], meta["xc"], meta["yc"], meta["probe_x"],
             meta["probe_y"], meta["probe_every"], meta["frame_every"], meta["steps"]) == (
         "cylinder", D, 5.0 * D, ny // 2 + 1, cs["probe_x"], cs["probe_y"], every, frame, steps)
+
+
+def test_reference_cylinder_run_reproduces_recorded_strouhal(lbm, driver, tmp_path, capsys):
+    """The reference's recorded cylinder runs (results/strouhal.csv: v1, v3 and v4, float,
+    1000 x 320, D 40, Re 100, u_in 0.05, 160,000 steps, probe every 50 steps, St over the last
+    45 %) through the driver and scripts/strouhal.py's analysis: the same St (FFT and zero
+    crossings), periods and u_y amplitude to the digits the reference records."""
+    from lbm_amd import formats
+
+    protocol = json.loads((pathlib.Path(__file__).parent / "golden" /
+                           "bench_protocol.json").read_text())
+    rec = [r for r in protocol["strouhal_rows"] if r["version"] in ("v1", "v3", "v4")]
+    assert len(rec) == 3 and len({(r["st_fft"], r["st_zero_crossing"], r["periods"],
+                                   r["uy_amplitude"]) for r in rec}) == 1
+    r = rec[0]
+    out = tmp_path / "cylinder"
+    print("driver cylinder 1000x320, 160000 steps", flush=True)
+    assert driver.run_driver(["--case", "cylinder", "--nx", r["nx"], "--ny", r["ny"],
+                              "--diameter", r["diameter"], "--re", r["re"], "--umax", r["u_in"],
+                              "--steps", r["steps"], "--out", str(out)]) == 0
+    capsys.readouterr()
+    a = formats.strouhal_analysis(out, start=int(r["window_start"]) / int(r["steps"]))
+    row = formats.strouhal_row("v5_hip", "v5_hip", a["meta"]["real"], a["meta"]["nx"],
+                               a["meta"]["ny"], a["meta"]["diameter"], a["meta"]["u_in"],
+                               a["meta"]["re"], a["meta"]["tau"], a["meta"]["steps"],
+                               a["window_start"], a["st_fft"], a["st_zero"], a["periods"],
+                               a["uy_amplitude"])
+    print({k: row[k] for k in ("tau", "window_start", "st_fft", "st_zero_crossing", "periods",
+                               "uy_amplitude")})
+    for k in ("real", "tau", "window_start", "st_fft", "st_zero_crossing", "periods",
+              "uy_amplitude"):
+        assert str(row[k]) == r[k], k
