@@ -340,3 +340,18 @@ def test_strouhal_analysis_of_a_written_probe(formats, lbm, tmp_path):
     st_fft, st_zero, periods = lbm.strouhal(uy[keep], float(every), D, U)
     assert abs(st_fft - a["st_fft"]) < 1e-4 and periods == a["periods"]
     assert abs(a["uy_amplitude"] - 0.03) < 1e-3
+
+
+def test_bandwidth_report_matches_ceiling_py_on_recorded_rows(lbm, capsys):
+    from lbm_amd import bandwidth
+
+    assert bandwidth.gpu_name("AMD Instinct MI355X (gfx950:sramecc+:xnack-), 256 CUs") == \
+        "AMD Instinct MI355X"
+    assert bandwidth.gpu_name(" (gfx950:sramecc+:xnack-), 256 CUs, 309.2 GB") == "AMD gfx950"
+    bandwidth.report(PROTOCOL["gpu_bandwidth_rows"], peak_gbs=320.0)
+    out = capsys.readouterr().out.splitlines()
+    # README.md:45-50: 231.9 GB/s measured, 3,221 MLUPS float ceiling
+    assert out[0] == "Measured best copy: 231.94 GB/s (Tesla T4)"
+    assert out[1].startswith("  float : measured ceiling =  3221.4 MLUPS")
+    assert out[2].startswith("  double: measured ceiling =  1610.7 MLUPS")
+    assert "hardware ceiling =  4444.4 MLUPS" in out[1]

@@ -209,3 +209,27 @@ def test_reference_cylinder_run_reproduces_recorded_strouhal(lbm, driver, tmp_pa
     for k in ("real", "tau", "window_start", "st_fft", "st_zero_crossing", "periods",
               "uy_amplitude"):
         assert str(row[k]) == r[k], k
+
+
+def test_bandwidth_tool_writes_gpu_bandwidth_csv(lbm, tmp_path, capsys):
+    import csv
+
+    from lbm_amd import bandwidth, formats
+
+    print("bandwidth tool", flush=True)
+    path = tmp_path / "gpu_bandwidth.csv"
+    for _ in range(2):  # the second run appends a row without a header
+        assert bandwidth.main([str(path), "--bytes", str(1 << 28), "--reps", "5",
+                               "--peak-gbs", "8000"]) == 0
+    text = capsys.readouterr().out
+    assert "Measured best copy:" in text and "float : measured ceiling" in text
+    with open(path, newline="") as f:
+        rd = csv.DictReader(f)
+        assert rd.fieldnames == formats.BANDWIDTH_COLUMNS
+        rows = list(rd)
+    assert len(rows) == 2
+    for r in rows:
+        assert float(r["copy_MBps"]) > 1e6 and float(r["memcpy_MBps"]) > 1e5  # > 1 TB/s
+        assert int(r["bytes_per_array"]) == 1 << 28 and int(r["reps"]) == 5
+    measured, hw = formats.ceiling_mlups(rows, 4, 8000.0)
+    assert 1e4 < measured < hw
