@@ -20,7 +20,10 @@ __device__ __forceinline__ uint32_t hash_cell(uint32_t c, uint32_t seed) {
 
 using half2_t = _Float16 __attribute__((ext_vector_type(2)));
 
-// fp32 (scaled units) -> binary16 bits, stochastic rounding driven by the high bits of r.
+// fp32 (scaled units) -> binary16 bits, stochastic rounding driven by the high bits of r
+// (bits 19-31 decide, measured). Each conversion clears a register for the half it keeps;
+// writing a second value into that half (the builtin's op_sel form) would save the clear, but
+// this compiler's placement of that half disagreed with the hardware's, so it is not used.
 __device__ __forceinline__ unsigned short to_half_sr(float v, uint32_t r) {
   const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{}, v, r, false);
   return __builtin_bit_cast(unsigned short, h[0]);
