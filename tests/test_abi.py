@@ -67,6 +67,8 @@ def test_null_handles_are_rejected(lbm):
     L = lbm.lib()
     assert L.lbm_step(None) == lbm.LBM_EINVAL
     assert L.lbm_run(None, 1) == lbm.LBM_EINVAL
+    assert L.lbm_vorticity(None, None, 1) == lbm.LBM_EINVAL
+    assert L.lbm_macroscopic(None, None, None, None) == lbm.LBM_EINVAL
     assert L.lbm_destroy(None) == lbm.LBM_OK
 
 
