@@ -233,3 +233,29 @@ def test_bandwidth_tool_writes_gpu_bandwidth_csv(lbm, tmp_path, capsys):
         assert int(r["bytes_per_array"]) == 1 << 28 and int(r["reps"]) == 5
     measured, hw = formats.ceiling_mlups(rows, 4, 8000.0)
     assert 1e4 < measured < hw
+
+
+def test_driver_fp16_storage_fields_close_to_fp64(lbm, oracle_mod, driver, tmp_path, capsys):
+    """--real fp16: the files are the float files of the reference's float build (float32 .npy,
+    meta real "float"), and the velocity is within the FP16 form's 1e-3 relative L2 of the
+    oracle's double run."""
+    nx, ny, steps = 64, 34, 3000
+    out = tmp_path / "fp16"
+    print("driver poiseuille fp16", flush=True)
+    assert driver.run_driver(["--nx", str(nx), "--ny", str(ny), "--steps", str(steps),
+                              "--tau", "1.0", "--out", str(out), "--real", "fp16"]) == 0
+    capsys.readouterr()
+    meta = json.loads((out / "meta.json").read_text())
+    assert meta["real"] == "float"
+    ux = np.load(out / "ux.npy", allow_pickle=False)
+    assert ux.dtype == np.float32 and ux.shape == (ny, nx)
+    force = lbm.poiseuille_force(ny - 2, 1.0, 0.05, "double")
+    o = oracle_mod.Oracle(nx, ny, 1.0, force, 0.0, oracle_mod.make_channel(nx, ny), "double")
+    o.init_equilibrium(1.0, 0.0, 0.0)
+    o.run(steps)
+    ref = o.macroscopic()[1].reshape(ny, nx)
+    err = np.sqrt(np.sum((ux.astype(np.float64) - ref) ** 2) / np.sum(ref ** 2))
+    print(f"fp16 driver u_x relative L2 vs fp64 oracle: {err:.2e}")
+    assert err < 1e-3
+    w = np.load(out / "vorticity.npy", allow_pickle=False)
+    assert w.dtype == np.float32 and np.all(w[0] == 0) and np.all(w[-1] == 0)
