@@ -284,7 +284,7 @@ def test_tile_orders_give_identical_bits(lbm, oracle_mod, prec, order):
 @pytest.mark.parametrize("grid,tile", [((53, 31), (512, 64)), ((53, 31), (64, 1)),
                                        ((200, 70), (64, 7)), ((640, 416), (512, 64)),
                                        ((640, 416), (128, 3)), ((1000, 300), (256, 16)),
-                                       # tiles taller than one 32-row halo chunk: several chunks,
+                                       # tiles taller than one halo chunk (16 rows): several chunks,
                                        # a partial last one, rows past the four-row trips
                                        ((640, 416), (512, 150)), ((333, 700), (128, 333)),
                                        # the automatic height
