@@ -176,7 +176,18 @@ double lbm_poiseuille_force(int precision, int H, double tau, double u_max);
  * {4,7,8} of the bottom owned row go down; 3 * nx reals per message. */
 typedef struct lbm_strip lbm_strip;
 int lbm_strip_create(int nx, int ny_local, int precision, lbm_strip** out);
+/* No reference counterpart: a strip with ghost_rows = 1 (as lbm_strip_create) or 2 ghost rows
+ * on each side; state [9][ny_local + 2 * ghost_rows][nx], owned rows are local rows ghost_rows
+ * .. ghost_rows + ny_local - 1 (ny_local >= ghost_rows). With two, one halo exchange serves two
+ * fused steps (lbm_strip_compute_rows2) and a message is 12 * nx reals: up, the 9 planes of the
+ * top owned row then planes {2,5,6} of the row below it (they land in ghost rows 1 and 0 of
+ * the strip above); down, the 9 planes of the bottom owned row then planes {4,7,8} of the row
+ * above it (ghost rows ny_local + 2 and ny_local + 3 of the strip below). Single steps
+ * (lbm_strip_compute_rows) work on either layout. */
+int lbm_strip_create_ghost(int nx, int ny_local, int precision, int ghost_rows, lbm_strip** out);
 int lbm_strip_destroy(lbm_strip* s);
+/* Reals per halo message: 3 * nx with one ghost row, 12 * nx with two. */
+int lbm_strip_halo_values(const lbm_strip* s, size_t* values);
 /* StripParams (strip_update.hpp:23-30): tau, force, open x with inlet velocity u_in. */
 int lbm_strip_set_params(lbm_strip* s, double tau, double force_x, double force_y, int open_x,
                          double u_in);
@@ -185,15 +196,27 @@ int lbm_strip_upload_state(lbm_strip* s, const void* g);
 int lbm_strip_download_state(lbm_strip* s, void* g);
 /* compute_rows(first, last, stream): local rows inclusive, stream 0 = boundary, 1 = interior. */
 int lbm_strip_compute_rows(lbm_strip* s, int first, int last, int stream);
-/* pack_halo: send_up / send_down (3 * nx reals each) from the new state; waits for stream 0. */
+/* No reference counterpart: two fused steps of owned local rows [first, last] in one launch of
+ * the two-step kernel (two ghost rows, periodic x, uniform force); it reads rows first - 2 ..
+ * last + 2 of the current state and writes rows first .. last of the next, bit-identical to
+ * two single steps. Disjoint row ranges may run on the two streams at once. */
+int lbm_strip_compute_rows2(lbm_strip* s, int first, int last, int stream);
+/* Tile of the two-step launches: block_x columns (64..512 float, ..256 double, multiple of 64;
+ * 0 keeps), rows (> 0 fixed, -1 automatic, 0 keeps), tile_order (0..2 as in
+ * lbm_set_kernel_options, -1 keeps; default 1). Same bits for every tile. */
+int lbm_strip_set_two_step_tile(lbm_strip* s, int block_x, int rows, int tile_order);
+/* pack_halo: send_up / send_down (lbm_strip_halo_values reals each) from the new state; waits
+ * for stream 0. */
 int lbm_strip_pack_halo(lbm_strip* s, void* send_up, void* send_down);
-/* unpack_halo: recv_from_down -> ghost row 0, recv_from_up -> ghost row ny_local + 1. */
+/* unpack_halo: recv_from_down -> the ghost rows below (row 0; rows 0 and 1 with two),
+ * recv_from_up -> the ghost rows above (row ny_local + 1; ny_local + 2 and + 3 with two). */
 int lbm_strip_unpack_halo(lbm_strip* s, const void* recv_from_down, const void* recv_from_up);
 /* finish_step: wait for both streams, swap the buffers. */
 int lbm_strip_finish_step(lbm_strip* s);
 int lbm_strip_set_block(lbm_strip* s, int block_x, int rows_per_thread);
-/* Seconds of `steps` rounds of compute_rows(1, ny_local) + ghost-row copy from a device halo
- * (no host staging), HIP events on stream 0: the one-GPU share at its real shape. */
+/* Seconds of `steps` rounds of compute_rows over the owned rows + ghost-row copy from a device
+ * halo (no host staging), HIP events on stream 0: the one-GPU share at its real shape. With two
+ * ghost rows and periodic x, pairs of steps take one compute_rows2 launch and one copy. */
 int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds);
 
 /* ---- batched persistent kernel over many independent domains ------------------------------- */

@@ -97,6 +97,10 @@ def _declare(L):
         "lbm_momentum_exchange": ([vp, i, i, i, i, dp, dp], i),
         "lbm_copy_ceiling": ([sz, i, dp, dp, dp, dp], i),
         "lbm_strip_create": ([i, i, i, C.POINTER(vp)], i),
+        "lbm_strip_create_ghost": ([i, i, i, i, C.POINTER(vp)], i),
+        "lbm_strip_halo_values": ([vp, C.POINTER(sz)], i),
+        "lbm_strip_compute_rows2": ([vp, i, i, i], i),
+        "lbm_strip_set_two_step_tile": ([vp, i, i, i], i),
         "lbm_strip_destroy": ([vp], i),
         "lbm_strip_set_params": ([vp, d, d, d, i, d], i),
         "lbm_strip_upload_solid": ([vp, _u8p], i),
@@ -446,15 +450,21 @@ HALO_DOWN = (4, 7, 8)  # c_y = -1: bottom owned row -> top ghost row of the stri
 
 class Strip:
     """StripEngine's interface (src/v4_mpi_cuda/engine.hpp:36-61) on the fused step: ny_local
-    owned rows plus ghost rows 0 and ny_local + 1, SoA state of 9 * nx * (ny_local + 2) reals."""
+    owned rows plus `ghost_rows` ghost rows below and above (1: the reference's strip; 2: one
+    halo exchange per two fused steps, compute_rows2), SoA state of 9 * nx * (ny_local +
+    2 * ghost_rows) reals. Owned rows are local rows ghost_rows .. ghost_rows + ny_local - 1."""
 
-    def __init__(self, nx, ny_local, precision="float"):
+    def __init__(self, nx, ny_local, precision="float", ghost_rows=1):
         self.precision = precision_code(precision)
         self.dtype = real_dtype(self.precision)
-        self.nx, self.ny_local = int(nx), int(ny_local)
+        self.nx, self.ny_local, self.ghost = int(nx), int(ny_local), int(ghost_rows)
         h = C.c_void_p()
-        check(lib().lbm_strip_create(self.nx, self.ny_local, self.precision, C.byref(h)))
+        check(lib().lbm_strip_create_ghost(self.nx, self.ny_local, self.precision, self.ghost,
+                                           C.byref(h)))
         self._h = h
+        n = C.c_size_t()
+        check(lib().lbm_strip_halo_values(self._h, C.byref(n)))
+        self.halo_values = n.value
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -466,8 +476,12 @@ class Strip:
         self.close()
 
     @property
+    def rows(self):
+        return self.ny_local + 2 * self.ghost
+
+    @property
     def N(self):
-        return self.nx * (self.ny_local + 2)
+        return self.nx * self.rows
 
     def set_params(self, tau, fx=0.0, fy=0.0, open_x=False, u_in=0.0):
         check(lib().lbm_strip_set_params(self._h, float(tau), float(fx), float(fy), int(open_x),
@@ -476,13 +490,13 @@ class Strip:
     def upload_solid(self, solid_strip):
         m = np.ascontiguousarray(solid_strip, dtype=np.uint8).reshape(-1)
         if m.size != self.N:
-            raise ValueError("strip mask must hold nx * (ny_local + 2) bytes")
+            raise ValueError("strip mask must hold nx * (ny_local + 2 * ghost_rows) bytes")
         check(lib().lbm_strip_upload_solid(self._h, m))
 
     def upload_state(self, g):
         a = np.ascontiguousarray(g, dtype=self.dtype).reshape(-1)
         if a.size != 9 * self.N:
-            raise ValueError("strip state must hold 9 * nx * (ny_local + 2) values")
+            raise ValueError("strip state must hold 9 * nx * (ny_local + 2 * ghost_rows) values")
         check(lib().lbm_strip_upload_state(self._h, a.ctypes.data))
 
     def download_state(self):
@@ -493,15 +507,25 @@ class Strip:
     def compute_rows(self, first, last, stream=0):
         check(lib().lbm_strip_compute_rows(self._h, int(first), int(last), int(stream)))
 
+    def compute_rows2(self, first, last, stream=0):
+        """Two fused steps of owned local rows [first, last] (two ghost rows, periodic x)."""
+        check(lib().lbm_strip_compute_rows2(self._h, int(first), int(last), int(stream)))
+
+    def set_two_step_tile(self, block_x=0, rows=0, tile_order=-1):
+        check(lib().lbm_strip_set_two_step_tile(self._h, int(block_x), int(rows),
+                                                int(tile_order)))
+
     def pack_halo(self):
-        up = np.empty(3 * self.nx, dtype=self.dtype)
-        down = np.empty(3 * self.nx, dtype=self.dtype)
+        up = np.empty(self.halo_values, dtype=self.dtype)
+        down = np.empty(self.halo_values, dtype=self.dtype)
         check(lib().lbm_strip_pack_halo(self._h, up.ctypes.data, down.ctypes.data))
         return up, down
 
     def unpack_halo(self, recv_from_down, recv_from_up):
-        a = np.ascontiguousarray(recv_from_down, dtype=self.dtype)
-        b = np.ascontiguousarray(recv_from_up, dtype=self.dtype)
+        a = np.ascontiguousarray(recv_from_down, dtype=self.dtype).reshape(-1)
+        b = np.ascontiguousarray(recv_from_up, dtype=self.dtype).reshape(-1)
+        if a.size != self.halo_values or b.size != self.halo_values:
+            raise ValueError(f"halo messages must hold {self.halo_values} values each")
         check(lib().lbm_strip_unpack_halo(self._h, a.ctypes.data, b.ctypes.data))
 
     def finish_step(self):

@@ -184,10 +184,50 @@ int step2_tile_rows(const StepParams<T>& p, int block_x);
 hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream);
 int step2_tile_rows_fp16(const StepParamsH& p, int block_x);
 
-// Strip ring closed on itself: ghost rows 0 and nyl + 1 of the state g (plane stride N, rows
-// of nx values) refreshed from the owned boundary rows, one launch.
+// ---- strips (one GPU's share of the v4 decomposition) ----------------------------------------
+// A strip holds nyl owned rows (local rows ghost .. ghost + nyl - 1) and `ghost` ghost rows on
+// each side. A halo message is a list of entries, each one row of nx values of one plane.
+//   ghost = 1 (src/v4_mpi_cuda/engine.hpp:17-20): 3 entries. Up: planes {2, 5, 6} (c_y = +1) of
+//     the top owned row, landing in ghost row 0 of the strip above; down: planes {4, 7, 8}
+//     (c_y = -1) of the bottom owned row, landing in ghost row nyl + 1 of the strip below.
+//   ghost = 2 (two steps per exchange): 12 entries. Up: the 9 planes of the top owned row (the
+//     receiver's ghost row 1), then planes {2, 5, 6} of the row under it (its ghost row 0);
+//     down: the 9 planes of the bottom owned row (the receiver's ghost row nyl + 2), then
+//     planes {4, 7, 8} of the row above it (its ghost row nyl + 3). Two fused steps read a
+//     strip's rows first - 2 .. last + 2: every plane of the inner ghost row (its own pulls and
+//     bounce-back), and of the outer one only the planes that cross into the inner one.
+__host__ __device__ constexpr int halo_entries(int ghost) { return ghost == 1 ? 3 : 12; }
+// Entry k of the message a strip sends up (up = 1) or down: its plane and local row there.
+__host__ __device__ inline void halo_send(int ghost, int nyl, int up, int k, int& plane, int& row) {
+  const int moving = up ? (k == 0 ? 2 : (k == 1 ? 5 : 6)) : (k == 0 ? 4 : (k == 1 ? 7 : 8));
+  if (ghost == 1) {
+    plane = moving;
+    row = up ? nyl : 1;
+  } else if (k < kQ) {
+    plane = k;
+    row = up ? nyl + 1 : 2;
+  } else {
+    const int j = k - kQ;
+    plane = up ? (j == 0 ? 2 : (j == 1 ? 5 : 6)) : (j == 0 ? 4 : (j == 1 ? 7 : 8));
+    row = up ? nyl : 3;
+  }
+}
+// Where entry k of a message received from the strip below (from_down = 1) or above lands.
+__host__ __device__ inline void halo_recv(int ghost, int nyl, int from_down, int k, int& plane,
+                                          int& row) {
+  int sent_row = 0;
+  halo_send(ghost, nyl, from_down, k, plane, sent_row);
+  if (ghost == 1)
+    row = from_down ? 0 : nyl + 1;
+  else
+    row = from_down ? (k < kQ ? 1 : 0) : (k < kQ ? nyl + 2 : nyl + 3);
+}
+
+// Strip ring closed on itself: the ghost rows of the state g (plane stride N, rows of nx
+// values) refreshed from its own owned rows, as if the strips above and below were this one
+// (the messages above, delivered on the device), one launch.
 template <class T>
-hipError_t launch_halo_ring(T* g, size_t N, int nx, int nyl, hipStream_t stream);
+hipError_t launch_halo_ring(T* g, size_t N, int nx, int nyl, int ghost, hipStream_t stream);
 
 // Grid-stride copy (HBM ceiling), n floats.
 // variant 0: scalar (the reference's kernel), 1: 16-byte accesses, 2: 16-byte non-temporal.

@@ -142,32 +142,36 @@ __global__ void __launch_bounds__(256) copy9_kernel(const float* __restrict__ a,
   }
 }
 
-// Ghost rows of a strip whose ring is closed on itself, in one launch: ghost row 0 gets planes
-// {2, 5, 6} (c_y = +1) of the top owned row ny_local, ghost row ny_local + 1 gets planes
-// {4, 7, 8} (c_y = -1) of the bottom owned row 1 (src/v4_mpi_cuda/engine.hpp:17-20): the
-// contents a halo exchange with the neighbouring strips delivers.
+// Ghost rows of a strip whose ring is closed on itself, in one launch: every entry of the
+// message the strip sends up lands where a message from the strip below would, and every
+// entry sent down where one from above would (halo_send / halo_recv): the contents a halo
+// exchange with the neighbouring strips delivers (src/v4_mpi_cuda/engine.hpp:17-20 for one
+// ghost row). blockIdx.y = direction * entries + entry; ghost rows are never read by a copy.
 template <class T>
 __global__ void __launch_bounds__(256) halo_ring_kernel(T* __restrict__ g, size_t N, int nx,
-                                                        int nyl) {
+                                                        int nyl, int ghost) {
   const int x = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
   if (x >= nx) return;
-  const int k = static_cast<int>(blockIdx.y);  // 0..5
-  const int plane = k < 3 ? (k == 0 ? 2 : (k == 1 ? 5 : 6)) : (k == 3 ? 4 : (k == 4 ? 7 : 8));
-  const size_t from = static_cast<size_t>(k < 3 ? nyl : 1) * nx + x;
-  const size_t to = static_cast<size_t>(k < 3 ? 0 : nyl + 1) * nx + x;
-  g[plane * N + to] = g[plane * N + from];
+  const int n = halo_entries(ghost);
+  const int up = static_cast<int>(blockIdx.y) < n ? 1 : 0;
+  const int k = static_cast<int>(blockIdx.y) - (up ? 0 : n);
+  int plane = 0, from = 0, to = 0;
+  halo_send(ghost, nyl, up, k, plane, from);
+  halo_recv(ghost, nyl, up, k, plane, to);  // sent up = received from below
+  g[plane * N + static_cast<size_t>(to) * nx + x] = g[plane * N + static_cast<size_t>(from) * nx + x];
 }
 
 }  // namespace
 
 template <class T>
-hipError_t launch_halo_ring(T* g, size_t N, int nx, int nyl, hipStream_t stream) {
-  const dim3 grid(static_cast<unsigned>((nx + 255) / 256), 6);
-  halo_ring_kernel<T><<<grid, 256, 0, stream>>>(g, N, nx, nyl);
+hipError_t launch_halo_ring(T* g, size_t N, int nx, int nyl, int ghost, hipStream_t stream) {
+  if ((ghost != 1 && ghost != 2) || nyl < ghost || nx < 1) return hipErrorInvalidValue;
+  const dim3 grid(static_cast<unsigned>((nx + 255) / 256), 2 * halo_entries(ghost));
+  halo_ring_kernel<T><<<grid, 256, 0, stream>>>(g, N, nx, nyl, ghost);
   return hipGetLastError();
 }
-template hipError_t launch_halo_ring<float>(float*, size_t, int, int, hipStream_t);
-template hipError_t launch_halo_ring<double>(double*, size_t, int, int, hipStream_t);
+template hipError_t launch_halo_ring<float>(float*, size_t, int, int, int, hipStream_t);
+template hipError_t launch_halo_ring<double>(double*, size_t, int, int, int, hipStream_t);
 
 __global__ void __launch_bounds__(kReduceThreads)
     reduce_final_kernel(const double* __restrict__ partials, int nblocks, double* __restrict__ out) {

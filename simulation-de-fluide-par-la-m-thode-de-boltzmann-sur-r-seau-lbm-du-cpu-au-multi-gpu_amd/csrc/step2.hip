@@ -187,12 +187,15 @@ constexpr int kHaloChunk = 16;
 constexpr int kHaloRows = kHaloChunk + 2;
 
 // The walk of one tile: columns x0 .. x0 + blockDim.x - 1 (those inside the grid), output rows
-// y0 .. y0 + H - 1 (those inside the grid). Ends with every LDS read done before the next walk's
+// y0 .. y0 + H - 1 (those below row_end). Ends with every LDS read done before the next walk's
 // first LDS write crosses a barrier (its halo chunk ends with one).
+// A whole periodic domain has row_end = ny; a strip's owned rows [row0, row_end) with two ghost
+// rows on each side (row0 >= 2, row_end <= ny - 2) read rows row0 - 2 .. row_end + 1 only, so
+// no row wraps and the ghost rows are plain inputs.
 template <class P, class PS, int kStoreAux>
 __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, const P& cells,
                                            const int x0, const int y0, const int H,
-                                           unsigned char* smem) {
+                                           const int row_end, unsigned char* smem) {
   using T = typename P::C;  // registers and LDS hold computed values
   using S = typename P::S;
   const int W = static_cast<int>(blockDim.x), WP = W + 2;
@@ -201,7 +204,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   const int t = static_cast<int>(threadIdx.x);
   const int nx = p.nx, ny = p.ny;
   const int wv = min(W, nx - x0);  // valid columns of this tile
-  const int y1 = min(y0 + H, ny);
+  const int y1 = min(y0 + H, row_end);
   const int nr = y1 - y0 + 2;                   // step-1 rows y0 - 1 .. y1
   PS g, gn;
   g.init(p.g, p.N);
@@ -364,7 +367,8 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
   const Step2View<T> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
   const PlainCells<T> cells{p.omega, p.fp, p.fx, p.fy};
   step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tx * static_cast<int>(blockDim.x),
-                                           ty * p.rows_per_tile, p.rows_per_tile, smem);
+                                           p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
+                                           p.row0 + p.nrows, smem);
 }
 
 // FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
@@ -378,7 +382,7 @@ __global__ void __launch_bounds__(512) step2_fp16_kernel(const StepParamsH p, co
   const Step2View<uint16_t> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
   const HalfCells cells{p.col, p.seed, seed2};
   step2_walk<HalfCells, PS, kAuxNone>(d, cells, tx * static_cast<int>(blockDim.x),
-                                      ty * p.rows_per_tile, p.rows_per_tile, smem);
+                                      ty * p.rows_per_tile, p.rows_per_tile, p.ny, smem);
 }
 
 // Batched domains: a grid of G workgroups walks the two-step work-list (domain, column tile, row
@@ -396,7 +400,7 @@ __global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p
                          p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg};
     const PlainCells<T> cells{dm.omega, dm.fp, dm.fx, dm.fy};
     step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tile.tx * static_cast<int>(blockDim.x),
-                                             tile.ty * p.tile_rows, p.tile_rows, smem);
+                                             tile.ty * p.tile_rows, p.tile_rows, dm.ny, smem);
   }
 }
 
@@ -416,9 +420,12 @@ size_t step2_lds_bytes(int block_x, size_t elem) {
 
 template <class T>
 hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream) {
-  // periodic, uniform force, whole domain, tile rows >= 1, a plane addressable with 32 bits
-  // and holding the discard slots
-  if (!p.wrap_y || p.force || p.row0 != 0 || p.nrows != p.ny || p.rows_per_tile < 1 ||
+  // uniform force; a whole periodic domain, or the owned rows of a strip with two ghost rows
+  // on each side; tile rows >= 1; a plane addressable with 32 bits and holding the discard
+  // slots
+  const bool whole = p.wrap_y && p.row0 == 0 && p.nrows == p.ny;
+  const bool strip = !p.wrap_y && p.row0 >= 2 && p.nrows >= 1 && p.row0 + p.nrows <= p.ny - 2;
+  if (!(whole || strip) || p.force || p.rows_per_tile < 1 ||
       block_x < 64 || block_x > 512 || block_x % 64 ||
       p.tiles_x != (p.nx + block_x - 1) / block_x ||
       static_cast<uint64_t>(p.N) * sizeof(T) >= (uint64_t{1} << 32) ||
@@ -426,7 +433,7 @@ hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream)
     return hipErrorInvalidValue;
   const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  const unsigned tiles_y = static_cast<unsigned>((p.ny + p.rows_per_tile - 1) / p.rows_per_tile);
+  const unsigned tiles_y = static_cast<unsigned>((p.nrows + p.rows_per_tile - 1) / p.rows_per_tile);
   if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
   const unsigned grid = static_cast<unsigned>(p.tiles_x) * tiles_y;
   const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
@@ -503,8 +510,8 @@ template <class T>
 int step2_tile_rows(const StepParams<T>& p, int block_x) {
   const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
-  return one ? tile_rows_for(step2_kernel<T, PlanesOne, kAuxNt>, p.nx, p.ny, block_x, lds)
-             : tile_rows_for(step2_kernel<T, Planes, kAuxNt>, p.nx, p.ny, block_x, lds);
+  return one ? tile_rows_for(step2_kernel<T, PlanesOne, kAuxNt>, p.nx, p.nrows, block_x, lds)
+             : tile_rows_for(step2_kernel<T, Planes, kAuxNt>, p.nx, p.nrows, block_x, lds);
 }
 template int step2_tile_rows<float>(const StepParams<float>&, int);
 template int step2_tile_rows<double>(const StepParams<double>&, int);
