@@ -118,6 +118,14 @@ int lbm_timed_run(lbm_solver* s, int steps, double* seconds);
 /* SolverV3::macroscopic — solver_v3.cu:322-343: rho, ux, uy (N reals each) from the
  * post-streaming state, computed on the device with v0's operations; solid cells get 0. */
 int lbm_macroscopic(lbm_solver* s, void* rho, void* ux, void* uy);
+/* No counterpart in the reference: lbm_macroscopic for callers that cannot allocate device
+ * memory (no hipMalloc once set up: tight field-output loops, harnesses that forbid it). Same
+ * outputs, bit for bit; the fields are staged in a fixed 12 MiB device array of the library
+ * (one per GPU, held until the process ends) instead of a buffer the handle allocates, so
+ * 3 * nx * ny reals must fit in 12 MiB (1,048,576 cells in float or FP16 storage, 524,288 in
+ * double), else LBM_EINVAL. Calls from one process are serialised; every call on a GPU reuses
+ * its array. */
+int lbm_macroscopic_noalloc(lbm_solver* s, void* rho, void* ux, void* uy);
 /* lbm::vorticity(ux, uy, solid, nx, ny, periodic_x) — src/common/io.cpp:54-74, applied to the
  * solver's macroscopic velocity: N reals (float for FP16 storage), 0 in solid cells, computed
  * on the device with the reference's operations. periodic_x: 1 periodic, 0 one-sided

@@ -80,6 +80,7 @@ def _declare(L):
         "lbm_run": ([vp, i], i),
         "lbm_timed_run": ([vp, i, dp], i),
         "lbm_macroscopic": ([vp, vp, vp, vp], i),
+        "lbm_macroscopic_noalloc": ([vp, vp, vp, vp], i),
         "lbm_vorticity": ([vp, vp, i], i),
         "lbm_total_mass": ([vp, dp], i),
         "lbm_momentum": ([vp, dp, dp], i),
@@ -312,6 +313,14 @@ class Solver:
     def macroscopic(self):
         r, u, v = (np.empty(self.N, dtype=self.dtype) for _ in range(3))
         check(lib().lbm_macroscopic(self._h, r.ctypes.data, u.ctypes.data, v.ctypes.data))
+        return r, u, v
+
+    def macroscopic_noalloc(self):
+        """macroscopic() without device allocation: the fields are staged in the library's
+        fixed 12 MiB device array (3 * nx * ny reals must fit; larger grids raise ValueError)."""
+        r, u, v = (np.empty(self.N, dtype=self.dtype) for _ in range(3))
+        check(lib().lbm_macroscopic_noalloc(self._h, r.ctypes.data, u.ctypes.data,
+                                            v.ctypes.data))
         return r, u, v
 
     def vorticity(self, periodic_x: int = -1) -> np.ndarray:

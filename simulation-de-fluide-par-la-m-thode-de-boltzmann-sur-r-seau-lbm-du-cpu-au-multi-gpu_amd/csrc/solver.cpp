@@ -25,8 +25,6 @@ namespace lbmhip {
 const char* last_error();
 }
 
-struct lbm_solver : lbmhip::SolverState {};
-
 namespace {
 
 int check_handle(const lbm_solver* s) {

@@ -191,6 +191,7 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
   block2_x = prec == LBM_DOUBLE ? 256 : 512;
   rows2 = -1;
   LBM_TRY(parse_block_env(&block_x, &rows));
+  LBM_HIP_TRY(hipGetDevice(&device));
   LBM_HIP_TRY(hipStreamCreate(&stream));
   LBM_HIP_TRY(hipEventCreate(&ev0));
   LBM_HIP_TRY(hipEventCreate(&ev1));
@@ -450,9 +451,14 @@ int SolverState::populations_at(int x, int y, void* out9) {
 
 int SolverState::macroscopic(void* rho, void* ux, void* uy) {
   if (!rho || !ux || !uy) return fail(LBM_EINVAL, "null output array");
+  LBM_TRY(ensure_staging(3 * cells() * real_size(prec)));
+  return macroscopic_into(staging, rho, ux, uy);
+}
+
+int SolverState::macroscopic_into(void* dev, void* rho, void* ux, void* uy) {
+  if (!rho || !ux || !uy) return fail(LBM_EINVAL, "null output array");
   const size_t rs = real_size(prec), n = cells();
-  LBM_TRY(ensure_staging(3 * n * rs));
-  char* st = static_cast<char*>(staging);
+  char* st = static_cast<char*>(dev);
   hipError_t e;
   if (prec == LBM_FLOAT)
     e = launch_macro(make_gather_params<float>(*this), reinterpret_cast<float*>(st),

@@ -10,6 +10,7 @@ namespace lbmhip {
 
 struct SolverState {
   int nx = 0, ny = 0, prec = 0;
+  int device = 0;  // the HIP device current at creation: it holds every buffer of the handle
   double tau = 0.0, fx = 0.0, fy = 0.0;  // as given; converted to the solver real at use
   std::vector<uint8_t> solid;
   bool open_x = false;
@@ -73,6 +74,9 @@ struct SolverState {
   int populations_at(int x, int y, void* out9);
   int set_populations(const void* in);
   int macroscopic(void* rho, void* ux, void* uy);
+  // The same into host arrays, through `dev`: a device buffer of at least 3 * nx * ny reals
+  // that receives the fields before the copies (the handle's staging buffer, or a caller's).
+  int macroscopic_into(void* dev, void* rho, void* ux, void* uy);
   int reduce(double (&out)[4]);
   int autotune(int steps, int* bx, int* rows, double* mlups);
   int momentum_exchange(int x0, int y0, int w, int h, double* fx, double* fy);
@@ -94,3 +98,6 @@ int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
                  double* reference_gbs, double* nine_gbs);
 
 }  // namespace lbmhip
+
+// The opaque handle of the C ABI (include/lbm_hip.h) is the solver state itself.
+struct lbm_solver : lbmhip::SolverState {};
