@@ -209,9 +209,18 @@ int lbm_strip_compute_rows(lbm_strip* s, int first, int last, int stream);
  * last + 2 of the current state and writes rows first .. last of the next, bit-identical to
  * two single steps. Disjoint row ranges may run on the two streams at once. */
 int lbm_strip_compute_rows2(lbm_strip* s, int first, int last, int stream);
+/* No reference counterpart: lbm_strip_compute_rows2 that also delivers the halo. As the
+ * launch stores the two outermost owned rows of each side of the new state, it stores their
+ * message entries straight into the next-state ghost rows of `up` (the strip above: its rows 0
+ * and 1) and `down` (the strip below: its rows ny_local + 2 and + 3), device to device, with
+ * no pack/unpack (the neighbours may be the strip itself). All strips need two ghost rows,
+ * the same nx and precision; a receiver's ghost rows are complete once the sender's streams
+ * are synchronised (its lbm_strip_finish_step). */
+int lbm_strip_compute_rows2_to(lbm_strip* s, int first, int last, int stream, lbm_strip* up,
+                               lbm_strip* down);
 /* Tile of the two-step launches: block_x columns (64..512 float, ..256 double, multiple of 64;
  * 0 keeps), rows (> 0 fixed, -1 automatic, 0 keeps), tile_order (0..2 as in
- * lbm_set_kernel_options, -1 keeps; default 1). Same bits for every tile. */
+ * lbm_set_kernel_options, -1 keeps; default 2). Same bits for every tile. */
 int lbm_strip_set_two_step_tile(lbm_strip* s, int block_x, int rows, int tile_order);
 /* pack_halo: send_up / send_down (lbm_strip_halo_values reals each) from the new state; waits
  * for stream 0. */
@@ -224,7 +233,8 @@ int lbm_strip_finish_step(lbm_strip* s);
 int lbm_strip_set_block(lbm_strip* s, int block_x, int rows_per_thread);
 /* Seconds of `steps` rounds of compute_rows over the owned rows + ghost-row copy from a device
  * halo (no host staging), HIP events on stream 0: the one-GPU share at its real shape. With two
- * ghost rows and periodic x, pairs of steps take one compute_rows2 launch and one copy. */
+ * ghost rows and periodic x, pairs of steps take one compute_rows2_to launch with the strip as
+ * its own neighbours (no copy). */
 int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds);
 
 /* ---- batched persistent kernel over many independent domains ------------------------------- */

@@ -101,6 +101,7 @@ def _declare(L):
         "lbm_strip_create_ghost": ([i, i, i, i, C.POINTER(vp)], i),
         "lbm_strip_halo_values": ([vp, C.POINTER(sz)], i),
         "lbm_strip_compute_rows2": ([vp, i, i, i], i),
+        "lbm_strip_compute_rows2_to": ([vp, i, i, i, vp, vp], i),
         "lbm_strip_set_two_step_tile": ([vp, i, i, i], i),
         "lbm_strip_destroy": ([vp], i),
         "lbm_strip_set_params": ([vp, d, d, d, i, d], i),
@@ -519,6 +520,12 @@ class Strip:
     def compute_rows2(self, first, last, stream=0):
         """Two fused steps of owned local rows [first, last] (two ghost rows, periodic x)."""
         check(lib().lbm_strip_compute_rows2(self._h, int(first), int(last), int(stream)))
+
+    def compute_rows2_to(self, first, last, stream, up: "Strip", down: "Strip"):
+        """compute_rows2 that also stores the halo entries straight into the next-state ghost
+        rows of the strips above (`up`) and below (`down`); no pack/unpack."""
+        check(lib().lbm_strip_compute_rows2_to(self._h, int(first), int(last), int(stream),
+                                               up._h, down._h))
 
     def set_two_step_tile(self, block_x=0, rows=0, tile_order=-1):
         check(lib().lbm_strip_set_two_step_tile(self._h, int(block_x), int(rows),

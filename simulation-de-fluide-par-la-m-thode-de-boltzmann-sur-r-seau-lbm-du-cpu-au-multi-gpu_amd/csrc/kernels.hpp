@@ -159,6 +159,21 @@ template <class S>
 hipError_t launch_exchange(const S* g, const uint16_t* links, size_t N, int nx, int x0, int y0,
                            int w, int h, float inv_scale, double* out, hipStream_t stream);
 
+// Halo entries delivered by a two-step strip launch itself: as each of the sender's two
+// outermost owned rows of the new state is stored, its message entries (halo_send) are also
+// stored into the receiving strips' next-state buffers at their halo_recv rows (on one GPU a
+// device store; strips on other GPUs would be reached through peer access). A solid cell's
+// entries go to the receiver's discard slots instead, so solid ghost cells keep their values.
+template <class T>
+struct HaloOut {
+  T* up;              // next-state buffer (plane 0) of the strip above: its ghost rows 0 and 1
+  T* down;            // ... of the strip below: its ghost rows down_row and down_row + 1
+  unsigned up_stride, down_stride;  // bytes between the receivers' planes
+  unsigned up_tail, down_tail;      // byte offset of each receiver's discard slots in a plane
+  int down_row;       // the receiver below's ny_local + 2
+  int lo, hi;         // the sender's first and last owned rows (local)
+};
+
 // Values every population plane holds past its nx * ny cells (plane stride N >= nx * ny +
 // kPlaneTail): launch_step2 stores the results of solid cells there instead of skipping them.
 constexpr size_t kPlaneTail = 64;
@@ -169,14 +184,20 @@ constexpr size_t kPlaneTail = 64;
 // launch_step (solid cells keep their values in both).
 template <class T>
 hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream);
+// The same over a strip's owned rows (wrap_y = 0), also storing its halo entries into the
+// receivers (HaloOut). Each receiver's nine planes must span < 4 GiB (one buffer resource).
+template <class T>
+hipError_t launch_step2_halo(const StepParams<T>& p, const HaloOut<T>& h, int block_x,
+                             hipStream_t stream);
 // LDS bytes per workgroup of launch_step2 (independent of the tile height); elem = bytes of the
 // computed type (4 for float and FP16 storage, 8 for double).
 size_t step2_lds_bytes(int block_x, size_t elem);
 // Tile height for launch_step2: the tallest of 64, 48, 32, 24, 16 rows whose tile count fills
 // the resident workgroup slots (occupancy of the instance p selects, on the current device)
-// evenly, else the best-filling one.
+// evenly with at least min_waves full waves (16 rows always qualifies), else the best-filling
+// one.
 template <class T>
-int step2_tile_rows(const StepParams<T>& p, int block_x);
+int step2_tile_rows(const StepParams<T>& p, int block_x, int min_waves = 1);
 
 // FP16 storage, two steps per launch (step2.hip): the conditions of launch_step2; p.seed is the
 // stochastic-rounding seed of the first step, seed2 of the second. Bit-identical to two

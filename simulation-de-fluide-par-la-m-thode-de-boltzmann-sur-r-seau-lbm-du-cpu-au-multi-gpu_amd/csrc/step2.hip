@@ -192,10 +192,11 @@ constexpr int kHaloRows = kHaloChunk + 2;
 // A whole periodic domain has row_end = ny; a strip's owned rows [row0, row_end) with two ghost
 // rows on each side (row0 >= 2, row_end <= ny - 2) read rows row0 - 2 .. row_end + 1 only, so
 // no row wraps and the ghost rows are plain inputs.
-template <class P, class PS, int kStoreAux>
+template <class P, class PS, int kStoreAux, bool kHalo = false>
 __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, const P& cells,
                                            const int x0, const int y0, const int H,
-                                           const int row_end, unsigned char* smem) {
+                                           const int row_end, unsigned char* smem,
+                                           const HaloOut<typename P::S>* hout = nullptr) {
   using T = typename P::C;  // registers and LDS hold computed values
   using S = typename P::S;
   const int W = static_cast<int>(blockDim.x), WP = W + 2;
@@ -319,6 +320,38 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
 #pragma unroll
       for (int i = 0; i < kQ; ++i) P::template store<kStoreAux>(gn, i, vs, so, out[i]);
+      if constexpr (kHalo) {
+        // the halo entries of the sender's two outermost owned rows on each side (q is
+        // wave-uniform: scalar branches, taken on four rows of a launch)
+        const HaloOut<S>& h = *hout;
+        if (q >= h.hi - 1 || q <= h.lo + 1) {
+          const bool solid = (prev_link & kLinkSolid) != 0;
+          const unsigned lane = static_cast<unsigned>(t & 63) * B;
+          auto put = [&](S* base, unsigned stride, unsigned tail, int row, int plane) {
+            const unsigned so_h = static_cast<unsigned>(row * nx) * B;
+            const unsigned vo_h = solid ? tail + lane - so_h : vo;
+            Stored<S>::template store<kAuxNone>(make_rsrc(base), vo_h,
+                                                so_h + static_cast<unsigned>(plane) * stride,
+                                                out[plane]);
+          };
+          if (q == h.hi) {  // all planes -> ghost row 1 of the strip above
+#pragma unroll
+            for (int i = 0; i < kQ; ++i) put(h.up, h.up_stride, h.up_tail, 1, i);
+          }
+          if (q == h.hi - 1) {  // c_y = +1 planes -> its ghost row 0
+#pragma unroll
+            for (int i : {2, 5, 6}) put(h.up, h.up_stride, h.up_tail, 0, i);
+          }
+          if (q == h.lo) {  // all planes -> the first ghost row above of the strip below
+#pragma unroll
+            for (int i = 0; i < kQ; ++i) put(h.down, h.down_stride, h.down_tail, h.down_row, i);
+          }
+          if (q == h.lo + 1) {  // c_y = -1 planes -> its outer ghost row
+#pragma unroll
+            for (int i : {4, 7, 8}) put(h.down, h.down_stride, h.down_tail, h.down_row + 1, i);
+          }
+        }
+      }
     }
     prev2 = prev[2];
 #pragma unroll
@@ -358,17 +391,17 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   }
 }
 
-template <class T, class PS, int kStoreAux>
-__global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p) {
+template <class T, class PS, int kStoreAux, bool kHalo = false>
+__global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p, const HaloOut<T> h) {
   extern __shared__ __align__(16) unsigned char smem[];
   const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const Step2View<T> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
   const PlainCells<T> cells{p.omega, p.fp, p.fx, p.fy};
-  step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tx * static_cast<int>(blockDim.x),
-                                           p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
-                                           p.row0 + p.nrows, smem);
+  step2_walk<PlainCells<T>, PS, kStoreAux, kHalo>(d, cells, tx * static_cast<int>(blockDim.x),
+                                                  p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
+                                                  p.row0 + p.nrows, smem, &h);
 }
 
 // FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
@@ -406,20 +439,17 @@ __global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p
 
 template <class T, class PS>
 void launch_planes(const StepParams<T>& p, unsigned grid, int block_x, size_t lds, hipStream_t stream) {
+  const HaloOut<T> none{};
   if (p.store_nt)
-    step2_kernel<T, PS, kAuxNt><<<grid, block_x, lds, stream>>>(p);
+    step2_kernel<T, PS, kAuxNt><<<grid, block_x, lds, stream>>>(p, none);
   else
-    step2_kernel<T, PS, kAuxNone><<<grid, block_x, lds, stream>>>(p);
+    step2_kernel<T, PS, kAuxNone><<<grid, block_x, lds, stream>>>(p, none);
 }
 
-}  // namespace
-
-size_t step2_lds_bytes(int block_x, size_t elem) {
-  return (static_cast<size_t>(kRingSlots) * kRingPlanes * (block_x + 2) + 2 * kHaloRows * 3) * elem;
-}
-
+// The checks of launch_step2 (one function for both entry points); returns the grid size, or 0
+// for an invalid configuration.
 template <class T>
-hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream) {
+unsigned step2_grid(const StepParams<T>& p, int block_x) {
   // uniform force; a whole periodic domain, or the owned rows of a strip with two ghost rows
   // on each side; tile rows >= 1; a plane addressable with 32 bits and holding the discard
   // slots
@@ -430,12 +460,24 @@ hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream)
       p.tiles_x != (p.nx + block_x - 1) / block_x ||
       static_cast<uint64_t>(p.N) * sizeof(T) >= (uint64_t{1} << 32) ||
       p.N < static_cast<size_t>(p.nx) * p.ny + kPlaneTail)
-    return hipErrorInvalidValue;
-  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
+    return 0;
+  if (step2_lds_bytes(block_x, sizeof(T)) > 64 * 1024) return 0;
   const unsigned tiles_y = static_cast<unsigned>((p.nrows + p.rows_per_tile - 1) / p.rows_per_tile);
-  if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
-  const unsigned grid = static_cast<unsigned>(p.tiles_x) * tiles_y;
+  if (p.xcd_bands == 2 && tiles_y % 8) return 0;
+  return static_cast<unsigned>(p.tiles_x) * tiles_y;
+}
+
+}  // namespace
+
+size_t step2_lds_bytes(int block_x, size_t elem) {
+  return (static_cast<size_t>(kRingSlots) * kRingPlanes * (block_x + 2) + 2 * kHaloRows * 3) * elem;
+}
+
+template <class T>
+hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream) {
+  const unsigned grid = step2_grid(p, block_x);
+  if (!grid) return hipErrorInvalidValue;
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
   if (one)
     launch_planes<T, PlanesOne>(p, grid, block_x, lds, stream);
@@ -445,6 +487,31 @@ hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream)
 }
 template hipError_t launch_step2<float>(const StepParams<float>&, int, hipStream_t);
 template hipError_t launch_step2<double>(const StepParams<double>&, int, hipStream_t);
+
+template <class T>
+hipError_t launch_step2_halo(const StepParams<T>& p, const HaloOut<T>& h, int block_x,
+                             hipStream_t stream) {
+  const unsigned grid = step2_grid(p, block_x);
+  // a strip (no y wrap) whose owned rows include h.lo .. h.hi's neighbours as the kernel sees
+  // them; every receiver's planes in one buffer resource
+  if (!grid || p.wrap_y || !h.up || !h.down || h.lo < 2 || h.hi < h.lo + 1 || h.hi > p.ny - 3 ||
+      h.down_row < 3 || h.up_tail < 2 * static_cast<unsigned>(p.nx) * sizeof(T) ||
+      h.down_tail < static_cast<unsigned>(h.down_row + 2) * static_cast<unsigned>(p.nx) * sizeof(T) ||
+      static_cast<uint64_t>(kQ) * h.up_stride >= (uint64_t{1} << 32) ||
+      static_cast<uint64_t>(kQ) * h.down_stride >= (uint64_t{1} << 32))
+    return hipErrorInvalidValue;
+  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
+  const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
+  if (one)
+    step2_kernel<T, PlanesOne, kAuxNt, true><<<grid, block_x, lds, stream>>>(p, h);
+  else
+    step2_kernel<T, Planes, kAuxNt, true><<<grid, block_x, lds, stream>>>(p, h);
+  return hipGetLastError();
+}
+template hipError_t launch_step2_halo<float>(const StepParams<float>&, const HaloOut<float>&, int,
+                                             hipStream_t);
+template hipError_t launch_step2_halo<double>(const StepParams<double>&, const HaloOut<double>&,
+                                              int, hipStream_t);
 
 template <class T>
 hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, int workgroups,
@@ -475,7 +542,7 @@ template hipError_t launch_batch_step2<double>(const BatchParams<double>&, int, 
 
 // Tile height for a two-step kernel instance: `kernel` with block_x threads and `lds` bytes.
 template <class K>
-int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds) {
+int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds, int min_waves = 1) {
   // Workgroups of this instance resident at once on the current device.
   int per_cu = 0, dev = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_x, lds) != hipSuccess ||
@@ -489,7 +556,9 @@ int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds) {
   // workgroups at least 97 % full), else the best-filling one. 8192^2 fp32, 512 columns, two
   // workgroups per CU: 64 rows (2048 tiles, four full waves) 146.3k MLUPS; 72 rows (1824 tiles,
   // last wave 56 % full) 125.8k, 56 rows 128.3k; 32 rows (eight waves, more recomputed rows)
-  // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k.
+  // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k. min_waves: candidates with
+  // fewer full waves are passed over unless none has that many (a strip's 8192 x 1024 owned
+  // rows: 16 rows, two waves, 136.2k MLUPS against 133.7k for 32 rows, one wave).
   const long slots = static_cast<long>(per_cu) * cus;
   const long tiles_x = (nx + block_x - 1) / block_x;
   int best = 16;
@@ -497,7 +566,7 @@ int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds) {
   for (const int rows : {64, 48, 32, 24, 16}) {
     const long tiles = tiles_x * ((ny + rows - 1) / rows);
     const double fill = static_cast<double>(tiles) / (((tiles + slots - 1) / slots) * slots);
-    if (fill >= 0.97) return rows;
+    if (fill >= 0.97 && (tiles >= min_waves * slots || rows == 16)) return rows;
     if (fill > best_fill) {
       best_fill = fill;
       best = rows;
@@ -507,14 +576,16 @@ int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds) {
 }
 
 template <class T>
-int step2_tile_rows(const StepParams<T>& p, int block_x) {
+int step2_tile_rows(const StepParams<T>& p, int block_x, int min_waves) {
   const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
-  return one ? tile_rows_for(step2_kernel<T, PlanesOne, kAuxNt>, p.nx, p.nrows, block_x, lds)
-             : tile_rows_for(step2_kernel<T, Planes, kAuxNt>, p.nx, p.nrows, block_x, lds);
+  return one ? tile_rows_for(step2_kernel<T, PlanesOne, kAuxNt>, p.nx, p.nrows, block_x, lds,
+                             min_waves)
+             : tile_rows_for(step2_kernel<T, Planes, kAuxNt>, p.nx, p.nrows, block_x, lds,
+                             min_waves);
 }
-template int step2_tile_rows<float>(const StepParams<float>&, int);
-template int step2_tile_rows<double>(const StepParams<double>&, int);
+template int step2_tile_rows<float>(const StepParams<float>&, int, int);
+template int step2_tile_rows<double>(const StepParams<double>&, int, int);
 
 namespace {
 bool fp16_one_resource(const StepParamsH& p) {

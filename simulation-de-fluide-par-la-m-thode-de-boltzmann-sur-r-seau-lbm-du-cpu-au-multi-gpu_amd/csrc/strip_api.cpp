@@ -35,8 +35,11 @@ struct lbm_strip {
   bool open_x = false;
   int block_x = 512, rows = 1;  // the plain solver's fp32/fp64 defaults (LBM_BLOCK overrides)
   // two-step tiles: 512 (float) / 256 (double) columns, height chosen by step2_tile_rows()
-  // when rows2 = -1, XCD-band tile order (the plain solver's two-step defaults)
-  int block2_x = 512, rows2 = -1, order2 = 1;
+  // with at least two full waves of tiles when rows2 = -1 (16 rows at 8192 x 1024); XCD-row
+  // tile order (whole tile rows per XCD; round-robin where the tile rows are not a multiple of
+  // 8). 8192 x 1024 fp32, 32-row tiles: 133.6k MLUPS against 130.6k with XCD bands, 131.1k
+  // round-robin; 16-row tiles 136.2k (XCD rows) and 133.0k (XCD bands)
+  int block2_x = 512, rows2 = -1, order2 = 2;
   mutable int rows2_auto = 0;  // the automatic height for the owned rows (0: not yet known)
   std::vector<uint8_t> solid;  // rows() * nx, ghost rows included
   size_t S = 0;                // plane stride (values) of the device buffers
@@ -177,7 +180,7 @@ struct lbm_strip {
       if (!rows2_auto) {
         StepParams<T> all = params<T>(ghost, ghost + nyl - 1);
         all.tiles_x = p.tiles_x;
-        rows2_auto = step2_tile_rows(all, block2_x);
+        rows2_auto = step2_tile_rows(all, block2_x, 2);
       }
       p.rows_per_tile = rows2_auto;
     }
@@ -211,6 +214,44 @@ struct lbm_strip {
     const hipError_t e = prec == LBM_DOUBLE
                              ? launch_step2(params2<double>(first, last), block2_x, streams[s])
                              : launch_step2(params2<float>(first, last), block2_x, streams[s]);
+    if (e != hipSuccess)
+      return fail(LBM_EHIP, std::string("strip two-step launch: ") + hipGetErrorString(e));
+    return LBM_OK;
+  }
+
+  // The halo targets of a two-step launch: the next-state buffers of the strips above and
+  // below (this strip itself when the ring is closed on it).
+  template <class T>
+  HaloOut<T> halo_to(const lbm_strip& up, const lbm_strip& down) const {
+    HaloOut<T> h{};
+    h.up = static_cast<T*>(up.gn);
+    h.down = static_cast<T*>(down.gn);
+    h.up_stride = static_cast<unsigned>(up.S * sizeof(T));
+    h.down_stride = static_cast<unsigned>(down.S * sizeof(T));
+    h.up_tail = static_cast<unsigned>(up.cells() * sizeof(T));
+    h.down_tail = static_cast<unsigned>(down.cells() * sizeof(T));
+    h.down_row = down.nyl + 2;
+    h.lo = ghost;
+    h.hi = ghost + nyl - 1;
+    return h;
+  }
+
+  int compute_rows2_to(int first, int last, int s, const lbm_strip& up, const lbm_strip& down) {
+    if (ghost != 2 || up.ghost != 2 || down.ghost != 2)
+      return fail(LBM_EINVAL, "halo delivery needs strips with two ghost rows");
+    if (up.nx != nx || down.nx != nx || up.prec != prec || down.prec != prec)
+      return fail(LBM_EINVAL, "halo delivery needs strips of the same width and precision");
+    if (open_x) return fail(LBM_EINVAL, "two steps per launch need a periodic x boundary");
+    if (last < first) return LBM_OK;
+    if (!owned(first, last) || s < 0 || s > 1)
+      return fail(LBM_EINVAL, "compute_rows2: rows must be owned local rows (2.." +
+                                  std::to_string(nyl + 1) + "), stream 0 or 1");
+    const hipError_t e =
+        prec == LBM_DOUBLE
+            ? launch_step2_halo(params2<double>(first, last), halo_to<double>(up, down), block2_x,
+                                streams[s])
+            : launch_step2_halo(params2<float>(first, last), halo_to<float>(up, down), block2_x,
+                                streams[s]);
     if (e != hipSuccess)
       return fail(LBM_EHIP, std::string("strip two-step launch: ") + hipGetErrorString(e));
     return LBM_OK;
@@ -305,6 +346,12 @@ int lbm_strip_compute_rows2(lbm_strip* s, int first, int last, int stream) {
   return s->compute_rows2(first, last, stream);
 }
 
+int lbm_strip_compute_rows2_to(lbm_strip* s, int first, int last, int stream, lbm_strip* up,
+                               lbm_strip* down) {
+  if (!s || !up || !down) return fail(LBM_EINVAL, "null strip handle");
+  return s->compute_rows2_to(first, last, stream, *up, *down);
+}
+
 int lbm_strip_set_two_step_tile(lbm_strip* s, int block_x, int rows, int tile_order) {
   if (!s) return fail(LBM_EINVAL, "null strip handle");
   const int max_x = s->prec == LBM_DOUBLE ? 256 : 512;
@@ -382,10 +429,12 @@ int lbm_strip_set_block(lbm_strip* s, int block_x, int rows_per_thread) {
 }
 
 int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds) {
-  // The strip at its real shape with the ring closed on itself: after each update the ghost
-  // rows of the new state are refreshed from the strip's own owned rows by one device kernel
-  // (the contents an RCCL/xGMI exchange would deliver), all on stream 0. With two ghost rows
-  // and a periodic x boundary, pairs of steps take one two-step launch and one refresh.
+  // The strip at its real shape with the ring closed on itself, all on stream 0: after each
+  // single step the ghost rows of the new state are refreshed from the strip's own owned rows
+  // by one device kernel (the contents an exchange with the neighbours would deliver); with two
+  // ghost rows and a periodic x boundary, pairs of steps take one two-step launch that stores
+  // its halo entries into its own next-state ghost rows (lbm_strip_compute_rows2_to with the
+  // strip as both neighbours): no refresh kernel.
   if (!s || steps < 0) return fail(LBM_EINVAL, "null strip handle or negative steps");
   hipStream_t st = s->streams[0];
   LBM_TRY(s->sync_all());
@@ -394,17 +443,19 @@ int lbm_strip_timed_steps(lbm_strip* s, int steps, double* seconds) {
   LBM_HIP_TRY(hipEventRecord(s->ev0, st));
   for (int k = 0; k < steps;) {
     const bool pair = two && steps - k >= 2;
-    hipError_t e;
-    if (pair)
-      e = s->prec == LBM_DOUBLE ? launch_step2(s->params2<double>(first, last), s->block2_x, st)
-                                : launch_step2(s->params2<float>(first, last), s->block2_x, st);
-    else
-      e = s->prec == LBM_DOUBLE
+    if (pair) {
+      LBM_TRY(s->compute_rows2_to(first, last, 0, *s, *s));
+    } else {
+      const hipError_t e =
+          s->prec == LBM_DOUBLE
               ? launch_step(s->params<double>(first, last), s->open_x, s->block_x, st)
               : launch_step(s->params<float>(first, last), s->open_x, s->block_x, st);
-    if (e != hipSuccess) return fail(LBM_EHIP, std::string("strip launch: ") + hipGetErrorString(e));
-    const hipError_t h = s->ring(st);
-    if (h != hipSuccess) return fail(LBM_EHIP, std::string("halo launch: ") + hipGetErrorString(h));
+      if (e != hipSuccess)
+        return fail(LBM_EHIP, std::string("strip launch: ") + hipGetErrorString(e));
+      const hipError_t h = s->ring(st);
+      if (h != hipSuccess)
+        return fail(LBM_EHIP, std::string("halo launch: ") + hipGetErrorString(h));
+    }
     std::swap(s->g, s->gn);
     k += pair ? 2 : 1;
   }

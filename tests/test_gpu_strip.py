@@ -42,14 +42,18 @@ def initial_pull_state(oracle_mod, nx, ny, tau, fx, fy, solid, prec, init, u_in=
     return o.raw_state()
 
 
-def boundary_then_interior(st, nyl, two, overlap):
+def boundary_then_interior(st, nyl, two, overlap, up=None, down=None):
     """SolverV4::step's launch order (solver_v4.cpp:119-123): the rows the neighbours need on
     stream 0, the interior on stream 1. A message of a strip with two ghost rows carries its
     two outermost owned rows on each side, so those are its boundary rows (for single steps
-    too): pack_halo waits for stream 0 only."""
+    too): pack_halo waits for stream 0 only. With `up` / `down` the two-step launches deliver
+    the halo into those strips themselves (compute_rows2_to)."""
     g = st.ghost
     lo, hi = g, g + nyl - 1
     run = st.compute_rows2 if two else st.compute_rows
+    if two and up is not None:
+        def run(a, b, s):
+            st.compute_rows2_to(a, b, s, up, down)
     depth = g
     if not overlap or nyl <= 2 * depth:
         run(lo, hi, 0)
@@ -60,9 +64,10 @@ def boundary_then_interior(st, nyl, two, overlap):
 
 
 def run_ring(lbm, nx, ny, heights, solid, R0, schedule, prec, tau, fx, fy, ghost,
-             open_x=False, u_in=0.0, overlap=True, tile=None):
+             open_x=False, u_in=0.0, overlap=True, tile=None, deliver=False):
     """A ring of strips stepped by `schedule` (a list of 1 / 2: steps per exchange); returns the
-    assembled pull state of the owned rows."""
+    assembled pull state of the owned rows. deliver: two-step rounds store the halo into the
+    neighbours' ghost rows from the kernel (no pack/unpack)."""
     assert sum(heights) == ny
     strips, starts = [], np.cumsum([0] + list(heights[:-1]))
     for y0, nyl in zip(starts, heights):
@@ -76,11 +81,17 @@ def run_ring(lbm, nx, ny, heights, solid, R0, schedule, prec, tau, fx, fy, ghost
         strips.append((st, y0, nyl))
     K = len(strips)
     for n in schedule:
-        for st, _, nyl in strips:
-            boundary_then_interior(st, nyl, n == 2, overlap)
-        halos = [st.pack_halo() for st, _, _ in strips]  # (send_up, send_down)
-        for k, (st, _, _) in enumerate(strips):
-            st.unpack_halo(halos[(k - 1) % K][0], halos[(k + 1) % K][1])
+        fused = deliver and n == 2
+        for k, (st, _, nyl) in enumerate(strips):
+            if fused:
+                boundary_then_interior(st, nyl, True, overlap, strips[(k + 1) % K][0],
+                                       strips[(k - 1) % K][0])
+            else:
+                boundary_then_interior(st, nyl, n == 2, overlap)
+        if not fused:
+            halos = [st.pack_halo() for st, _, _ in strips]  # (send_up, send_down)
+            for k, (st, _, _) in enumerate(strips):
+                st.unpack_halo(halos[(k - 1) % K][0], halos[(k + 1) % K][1])
         for st, _, _ in strips:
             st.finish_step()
     out = np.empty((9, ny, nx), dtype=R0.dtype)
@@ -116,6 +127,21 @@ def test_ring_of_strips_matches_reference_v0(lbm, oracle_mod, prec, ghost, heigh
                  p["tau"], p["fx"], p["fy"], ghost, overlap=len(heights) % 2 == 0)
     S = oracle_mod.pull_to_aos(R, nx, ny, solid)
     assert_bitwise(S, ref, f"{prec} ghost {ghost} strips {heights}")
+
+
+@pytest.mark.parametrize("prec", ("float", "double"))
+@pytest.mark.parametrize("heights", HEIGHTS[2])
+def test_kernel_delivered_halo_matches_reference_v0(lbm, oracle_mod, prec, heights):
+    """Two ghost rows, halo stored by each two-step launch straight into the neighbours' next
+    ghost rows (strips of uneven height receive at different rows; one strip is its own
+    neighbour), mixed with single steps exchanged through pack/unpack: golden v0 bits."""
+    print(f"strip ring delivered halo {prec} heights {heights}", flush=True)
+    solid, ref, p = load_golden(f"walls_block_53x31_500_{prec}.npz")
+    nx, ny = p["nx"], p["ny"]
+    R0 = initial_pull_state(oracle_mod, nx, ny, p["tau"], p["fx"], p["fy"], solid, prec, p["init"])
+    R = run_ring(lbm, nx, ny, heights, solid, R0, schedule_for(p["steps"], 2, mixed=True), prec,
+                 p["tau"], p["fx"], p["fy"], 2, overlap=len(heights) % 2 == 0, deliver=True)
+    assert_bitwise(oracle_mod.pull_to_aos(R, nx, ny, solid), ref, f"{prec} delivered {heights}")
 
 
 @pytest.mark.parametrize("prec", ("float", "double"))
@@ -200,6 +226,15 @@ def test_strip_rejects_bad_calls(lbm):
     st.set_params(0.8, 0.0, 0.0, True, 0.05)
     with pytest.raises(ValueError, match="periodic x"):
         st.compute_rows2(2, 9)
+    other = lbm.Strip(64, 8, "double", 1)
+    with pytest.raises(ValueError, match="two ghost rows"):
+        st.compute_rows2_to(2, 9, 0, other, st)
+    other.close()
+    wide = lbm.Strip(128, 8, "double", 2)
+    st.set_params(0.8)
+    with pytest.raises(ValueError, match="same width"):
+        st.compute_rows2_to(2, 9, 0, st, wide)
+    wide.close()
     st.close()
     with pytest.raises(ValueError):
         lbm.Strip(64, 1, "float", 2)  # fewer owned rows than ghost rows
