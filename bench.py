@@ -18,8 +18,10 @@ Besides the headline value the line carries:
                 the vector-ALU issue ceiling (256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
                 instruction) over the measured launch time: the kernel's other bound;
   cpu_baseline  the reference's own v1 OpenMP solver compiled from its sources (oracle/_ref),
-                or the oracle's OpenMP port when that is absent, timed on this host on a
-                bounded sample of the same channel;
+                or the oracle's OpenMP port when that is absent, timed on this host on the
+                reference's protocol (warm-up steps / 5, 5 runs, mean / std / max) at one
+                thread and at all physical cores of the process's CPU share, with the CPU
+                model from lscpu; `value` is the all-core mean on the 8192^2 channel;
   forms         the other measured forms (one step per launch, the reference's 5-repetition
                 protocol on the 8192^2 and 2048^2 instances, copy ceiling, fp64, FP16 storage,
                 batched domains, strip share, adjoint cost, cylinder wake).
@@ -31,6 +33,7 @@ import json
 import os
 import pathlib
 import re
+import subprocess
 import sys
 import time
 
@@ -77,25 +80,78 @@ def run_plain(lbm, nx, ny, steps, warmup, precision="float", block=None, steps_p
     return t, info, mass
 
 
-def cpu_baseline(nx, ny, steps=10, warmup=2):
-    """Reference v1 (oracle/_ref) or the oracle's OpenMP port, on this host's cores."""
-    import oracle
+def host_cpu():
+    """The host's CPU as run_bench.py records it (bench/run_bench.py:135-160): model, sockets,
+    cores and threads per core from lscpu, the CPUs this process may run on, the frequency
+    governor, and the physical cores of that share."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        out = ""
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    gov = pathlib.Path("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor")
+    info["governor"] = gov.read_text().strip() if gov.exists() else "unknown"
+    info["process_cpus"] = len(os.sched_getaffinity(0))
+    info["process_physical_cores"] = max(1, info["process_cpus"] // int(info.get("threads_per_core") or 1))
+    return info
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    sample = f"{nx}x{ny} walled channel, tau 0.6, gx 1e-6, {warmup} untimed + {steps} timed steps"
-    if oracle.Reference.available("float"):
-        ref = oracle.Reference("float")
-        sec, used = ref.v1_channel_timed(nx, ny, 0.6, 1e-6, warmup, steps)
-        kind = "reference"
-    else:
-        o = oracle.Oracle(nx, ny, 0.6, 1e-6, 0.0, oracle.make_channel(nx, ny), "float", "pull")
-        o.init_equilibrium(1.0, 0.0, 0.0)
-        o.run(warmup)
-        sec = o.timed_run(steps)
-        used = o.threads()
-        kind = "port"
-    return {"value": nx * ny * steps / sec / 1e6, "unit": "MLUPS", "cores": used or threads,
-            "kind": kind, "sample": sample, "seconds": sec}
+
+# CPU baseline instances (BASELINE.md's CPU-baseline plan): the 2048^2 throughput instance of the
+# plain method (walls + seeded 8x8 block, tau 0.8, F = (1e-5, 2e-6)) and the 8192^2 channel
+# (tau 0.6, gx 1e-6), rest init, float. Timed steps per repetition: the protocol's own for all
+# cores; bounded samples for one thread (the full protocol there would take minutes).
+CPU_INSTANCES = (
+    ("block_2048", 2048, ["--mask", "block", "--tau", "0.8", "--fx", "1e-5", "--fy", "2e-6"],
+     {"all": 1000, "one": 100}),
+    ("channel_8192", 8192, ["--mask", "channel", "--tau", "0.6", "--fx", "1e-6"],
+     {"all": 200, "one": 10}),
+)
+
+
+def cpu_baseline(full: bool = False):
+    """The reference's v1 (oracle/_ref) or the oracle's OpenMP port on this host, on the
+    reference's protocol: warm-up of steps / 5, then 5 timed runs in one process, mean / std /
+    max (src/common/driver.hpp:115-132, bench/run_bench.py:124-160), with OMP_PROC_BIND=close
+    and OMP_PLACES=cores, at 1 thread and at all physical cores of this process's CPU share
+    (the OMP_NUM_THREADS the host sets, when it sets one). Each run is a child process
+    (oracle/cpu_protocol.py) so the placement is in force when OpenMP starts. `value` is the
+    all-core rate on the headline's 8192^2 channel."""
+    cpu = host_cpu()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(cpu["process_physical_cores"], share) if share else cpu["process_physical_cores"]
+    runs = {}
+    for name, n, extra, steps in CPU_INSTANCES:
+        for label, threads in (("one_thread", 1), ("all_cores", cores)):
+            k = steps["all"] if (label == "all_cores" or full) else steps["one"]
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores")
+            cmd = [sys.executable, "-m", "oracle.cpu_protocol", "--nx", str(n), "--ny", str(n),
+                   "--steps", str(k), "--reps", "5", *extra]
+            t0 = time.time()
+            r = subprocess.run(cmd, cwd=str(ROOT), env=env, capture_output=True, text=True,
+                               timeout=1800)
+            if r.returncode != 0:
+                raise RuntimeError(f"cpu protocol {name} {label}: {r.stderr[-500:]}")
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            runs[f"{name}_{label}"] = res
+            log(f"cpu {name} {label} ({res['threads']} threads, {k} steps x 5): "
+                f"{res['mean']:.1f} +- {res['std']:.1f} MLUPS ({time.time() - t0:.1f} s)")
+    head = runs["channel_8192_all_cores"]
+    return {"value": head["mean"], "unit": "MLUPS", "cores": head["threads"], "kind": head["kind"],
+            "sample": f"8192x8192 walled channel, tau 0.6, gx 1e-6, float: warm-up "
+                      f"{head['warmup']} steps, then 5 runs of {head['steps']} steps (mean; "
+                      f"reference protocol), {head['threads']} OpenMP threads, "
+                      f"OMP_PROC_BIND=close, OMP_PLACES=cores",
+            "std": head["std"], "max": head["max"], "cpu": cpu, "runs": runs,
+            "note": "one-thread runs time 100 (2048^2) and 10 (8192^2) steps per repetition "
+                    "(bounded sample) unless --cpu-full"}
 
 
 def form_single_step(lbm, args):
@@ -341,8 +397,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms")
     ap.add_argument("--forms", default="all",
                     help="comma-separated subset of: " + ",".join(FORMS) + " (default all)")
-    ap.add_argument("--cpu-steps", type=int, default=400,
-                    help="timed steps of the CPU-baseline sample (8192^2 cells each)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline: the full protocol steps at one thread too (slow)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: every box here has a single MI355X and nothing in "
@@ -422,9 +478,13 @@ def main():
                     pr["plain_fp16"]["mlups"] / ss["mlups"], 3)
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(args.nx, args.ny, steps=args.cpu_steps)
-        log(f"cpu baseline: {line['cpu_baseline']['value']:.1f} MLUPS on "
-            f"{line['cpu_baseline']['cores']} threads ({line['cpu_baseline']['kind']})")
+        try:
+            line["cpu_baseline"] = cpu_baseline(args.cpu_full)
+            log(f"cpu baseline: {line['cpu_baseline']['value']:.1f} MLUPS on "
+                f"{line['cpu_baseline']['cores']} threads ({line['cpu_baseline']['kind']})")
+        except Exception as e:  # the CPU leg failing must not lose the headline line
+            line["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"cpu baseline failed: {e}")
     print(json.dumps(line), flush=True)
 
 

@@ -86,6 +86,8 @@ class SolverHip {
   void set_tau(real tau) { check(lbm_set_tau(h_, tau)); }
   void set_force(real fx, real fy) { check(lbm_set_force(h_, fx, fy)); }
   void set_mask(Mask solid) {
+    // the C ABI reads nx * ny bytes (the constructor's check, solver_v3.cu:169-170)
+    if (solid.size() != cells()) throw std::invalid_argument("solid mask size does not match the grid");
     check(lbm_set_mask(h_, solid.data()));
     solid_ = std::move(solid);
   }

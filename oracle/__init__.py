@@ -4,7 +4,7 @@ ctypes bindings for the CPU restatement of the reference solver (``lbm_oracle.cp
 ``oracle/build/liblbm_oracle.so``) and for the reference's own CPU solvers compiled from their
 sources into ``oracle/_ref/`` (``ref_shim.cpp``). Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` import this package, and only as a checker or a timed
-CPU baseline.
+CPU baseline (``cpu_protocol.py``: the reference's timing protocol in a child process).
 
 Function by function, what pins it (besides the reference's own outputs in tests/golden/):
   * ``equilibrium`` / ``guo`` — closed-form moment identities (tests/test_oracle.py);
@@ -271,6 +271,7 @@ class Reference:
         L.ref_make_cylinder.argtypes = [i, i, d, d, d, _u8p]
         L.ref_v1_channel_timed.argtypes = [i, i, d, d, i, i, C.POINTER(i)]
         L.ref_v1_channel_timed.restype = d
+        L.ref_v1_protocol.argtypes = [i, i, d, d, d, _u8p, i, i, i, C.POINTER(d), C.POINTER(i)]
         L.ref_real_size.restype = i
         L.ref_vorticity.argtypes = [vp, vp, _u8p, i, i, i, vp]
         L.ref_write_npy.argtypes = [C.c_char_p, vp, i, i, i]
@@ -307,6 +308,16 @@ class Reference:
         if kind == 0:
             a = np.ascontiguousarray(a, dtype=self.dtype)
         self.L.ref_write_npy(str(path).encode(), a.ctypes.data, int(ny), int(nx), kind)
+
+    def v1_protocol(self, nx, ny, tau, fx, fy, solid, warmup, steps, reps):
+        """Seconds of `reps` timed runs of `steps` steps of the reference's v1 after `warmup`
+        untimed steps (one solver, rest start), and the OpenMP threads it used."""
+        secs = (C.c_double * reps)()
+        t = C.c_int()
+        self.L.ref_v1_protocol(nx, ny, float(tau), float(fx), float(fy),
+                               np.ascontiguousarray(solid, np.uint8).reshape(-1), int(warmup),
+                               int(steps), int(reps), secs, C.byref(t))
+        return list(secs), t.value
 
     def v1_channel_timed(self, nx, ny, tau, fx, warmup, steps):
         t = C.c_int()

@@ -85,6 +85,24 @@ double ref_v1_channel_timed(int nx, int ny, double tau, double fx, int warmup, i
   return t;
 }
 
+// The reference's timing protocol (src/common/driver.hpp:115-132) on SolverV1: one solver on
+// the given mask, rest initial state, `warmup` untimed steps, then `reps` timed runs of `steps`
+// steps (host clock around run(), as driver.hpp's timed() does for a solver without
+// timed_run); seconds[r] receives run r's time.
+void ref_v1_protocol(int nx, int ny, double tau, double fx, double fy, const std::uint8_t* solid,
+                     int warmup, int steps, int reps, double* seconds, int* threads) {
+  lbm::Mask m(solid, solid + static_cast<std::size_t>(nx) * ny);
+  lbm::SolverV1 s(nx, ny, static_cast<real>(tau), static_cast<real>(fx), static_cast<real>(fy), m);
+  s.init_equilibrium(real(1), real(0), real(0));
+  s.run(warmup);
+  for (int r = 0; r < reps; ++r) {
+    const auto t0 = std::chrono::steady_clock::now();
+    s.run(steps);
+    seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (threads) *threads = s.threads();
+}
+
 // io.cpp's vorticity (src/common/io.cpp:54-74) of caller-owned ux, uy (nx * ny reals).
 void ref_vorticity(const void* ux, const void* uy, const std::uint8_t* solid, int nx, int ny,
                    int periodic_x, void* out) {

@@ -528,17 +528,11 @@ int lbm_batch_total_cells(const lbm_batch* b, long long* cells) {
   return LBM_OK;
 }
 
-int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
-                       double* best_mlups) {
-  // Times `steps` direct launches per candidate shape on a copy of the state (restored after);
-  // every shape computes the same bits.
-  if (!b || steps < 1) return fail(LBM_EINVAL, "null batch handle or steps < 1");
-  const size_t bytes = b->total_elems * b->rs();
-  void* backup = nullptr;
-  if (hipMalloc(&backup, bytes) != hipSuccess) return fail(LBM_ENOMEM, "autotune backup allocation failed");
-  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
-  LBM_HIP_TRY(hipMemcpy(backup, b->current(), bytes, hipMemcpyDeviceToDevice));
-  const int keep_parity = b->parity;
+namespace {
+
+// The timed trials of lbm_batch_autotune (the state is saved and restored around them).
+int batch_autotune_trials(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
+                          double* best_mlups) {
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -624,14 +618,53 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int
     LBM_TRY(b->build_tiles2());
     if (best2 > 0.0) best = best2;
   }
-  b->parity = keep_parity;
-  LBM_HIP_TRY(hipMemcpy(b->current(), backup, bytes, hipMemcpyDeviceToDevice));
-  LBM_HIP_TRY(hipFree(backup));
   if (tile_x) *tile_x = btx;
   if (tile_rows) *tile_rows = brows;
   if (workgroups) *workgroups = bwg;
   if (best_mlups) *best_mlups = best;
   return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_batch_autotune(lbm_batch* b, int steps, int* tile_x, int* tile_rows, int* workgroups,
+                       double* best_mlups) {
+  // Times `steps` direct launches per candidate shape on a copy of the state, which is restored
+  // afterwards whatever happens; when a trial fails, the launch configuration is restored too.
+  // Every shape computes the same bits.
+  if (!b || steps < 1) return fail(LBM_EINVAL, "null batch handle or steps < 1");
+  const size_t bytes = b->total_elems * b->rs();
+  void* backup = nullptr;
+  if (hipMalloc(&backup, bytes) != hipSuccess) return fail(LBM_ENOMEM, "autotune backup allocation failed");
+  const int keep_parity = b->parity;
+  const int k_tx = b->tile_x, k_rows = b->tile_rows, k_wg = b->workgroups, k_order = b->order,
+            k_nt = b->store_nt, k_t2x = b->tile2_x, k_t2r = b->tile2_rows, k_wg2 = b->workgroups2;
+  const bool saved = hipStreamSynchronize(b->stream) == hipSuccess &&
+                     hipMemcpy(backup, b->current(), bytes, hipMemcpyDeviceToDevice) == hipSuccess;
+  int rc = saved ? batch_autotune_trials(b, steps, tile_x, tile_rows, workgroups, best_mlups)
+                 : fail(LBM_EHIP, "autotune: saving the state failed");
+  b->parity = keep_parity;
+  bool restored = !saved;
+  if (saved) {
+    (void)hipGetLastError();  // a failed trial's launch error must not fail the restore
+    restored = hipStreamSynchronize(b->stream) == hipSuccess &&
+               hipMemcpy(b->current(), backup, bytes, hipMemcpyDeviceToDevice) == hipSuccess;
+  }
+  (void)hipFree(backup);
+  if (rc != LBM_OK) {
+    b->tile_x = k_tx;
+    b->tile_rows = k_rows;
+    b->workgroups = k_wg;
+    b->order = k_order;
+    b->store_nt = k_nt;
+    b->tile2_x = k_t2x;
+    b->tile2_rows = k_t2r;
+    b->workgroups2 = k_wg2;
+    // the lists of the restored shapes (a failure here is secondary to the trial's)
+    if (b->build_tiles() == LBM_OK) (void)b->build_tiles2();
+    return rc;
+  }
+  return restored ? LBM_OK : fail(LBM_EHIP, "autotune: restoring the state failed");
 }
 
 }  // extern "C"

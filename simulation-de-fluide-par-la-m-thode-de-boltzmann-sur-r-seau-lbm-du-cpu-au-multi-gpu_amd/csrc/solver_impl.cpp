@@ -552,16 +552,51 @@ int SolverState::momentum_exchange(int x0, int y0, int w, int h, double* fx, dou
 }
 
 int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_mlups) {
-  // Trials run on a copy of the state, which is restored afterwards. Every launch shape
-  // computes the same bits (each cell's arithmetic does not depend on the shape).
+  // Trials run on a copy of the state, which is restored afterwards whatever happens; when a
+  // trial fails, the launch configuration is restored too. Every launch shape computes the
+  // same bits (each cell's arithmetic does not depend on the shape).
   const size_t bytes = state_bytes();
   void* backup = nullptr;
   if (hipMalloc(&backup, bytes) != hipSuccess)
     return fail(LBM_ENOMEM, "device allocation of the autotune backup failed");
-  LBM_HIP_TRY(hipStreamSynchronize(stream));
-  LBM_HIP_TRY(hipMemcpy(backup, g, bytes, hipMemcpyDeviceToDevice));
-  const int keep_bx = block_x, keep_rows = rows;
+  const int k_bx = block_x, k_rows = rows, k_nt = store_nt, k_one = one_rsrc, k_band = xcd_bands,
+            k_b2x = block2_x, k_r2 = rows2, k_r2a = rows2_auto, k_o2 = two_step_order;
   const uint32_t keep_count = step_count;  // FP16 rounding seeds stay as if no trial ran
+  bool saved = false;
+  int rc = hipStreamSynchronize(stream) == hipSuccess &&
+                   hipMemcpy(backup, g, bytes, hipMemcpyDeviceToDevice) == hipSuccess
+               ? LBM_OK
+               : fail(LBM_EHIP, "autotune: saving the state failed");
+  if (rc == LBM_OK) {
+    saved = true;
+    rc = autotune_trials(steps, best_bx, best_rows, best_mlups);
+  }
+  if (rc != LBM_OK) {
+    block_x = k_bx;
+    rows = k_rows;
+    store_nt = k_nt;
+    one_rsrc = k_one;
+    xcd_bands = k_band;
+    block2_x = k_b2x;
+    rows2 = k_r2;
+    rows2_auto = k_r2a;
+    two_step_order = k_o2;
+  }
+  step_count = keep_count;
+  bool restored = !saved;
+  if (saved) {
+    (void)hipGetLastError();  // a failed trial's launch error must not fail the restore
+    restored = hipStreamSynchronize(stream) == hipSuccess &&
+               hipMemcpy(g, backup, bytes, hipMemcpyDeviceToDevice) == hipSuccess &&
+               hipMemcpy(gn, backup, bytes, hipMemcpyDeviceToDevice) == hipSuccess;
+  }
+  (void)hipFree(backup);
+  if (rc != LBM_OK) return rc;
+  return restored ? LBM_OK : fail(LBM_EHIP, "autotune: restoring the state failed");
+}
+
+int SolverState::autotune_trials(int steps, int* best_bx, int* best_rows, double* best_mlups) {
+  const int keep_bx = block_x, keep_rows = rows;
   double best = -1.0;
   int bbx = keep_bx, brows = keep_rows, bnt = store_nt, bone = one_rsrc, bband = xcd_bands;
   for (int band : {0, 1, 2})
@@ -633,10 +668,6 @@ int SolverState::autotune(int steps, int* best_bx, int* best_rows, double* best_
     two_step_order = b2o;
     if (best2 > 0.0) best = best2;
   }
-  step_count = keep_count;
-  LBM_HIP_TRY(hipMemcpy(g, backup, bytes, hipMemcpyDeviceToDevice));
-  LBM_HIP_TRY(hipMemcpy(gn, backup, bytes, hipMemcpyDeviceToDevice));
-  LBM_HIP_TRY(hipFree(backup));
   if (best_bx) *best_bx = bbx;
   if (best_rows) *best_rows = brows;
   if (best_mlups) *best_mlups = best;

@@ -82,6 +82,7 @@ struct SolverState {
   int momentum_exchange(int x0, int y0, int w, int h, double* fx, double* fy);
 
  private:
+  int autotune_trials(int steps, int* bx, int* rows, double* mlups);  // autotune's timed trials
   int upload_links();
   int ensure_staging(size_t bytes);
   size_t staging_bytes = 0;
