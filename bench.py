@@ -32,7 +32,6 @@ import argparse
 import json
 import os
 import pathlib
-import re
 import subprocess
 import sys
 import time
@@ -44,21 +43,26 @@ import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"float": 72, "double": 144, "fp16": 36}
-PROFILE_SUMMARY = ROOT / "profiles" / "step2_fp32_8192x8192.md"
+TRAFFIC_SUMMARY = ROOT / "profiles" / "step2_fp32_8192x8192.traffic.md"
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def from_profile(key):
-    """A per-launch figure of the fp32 two-step kernel from the committed rocprofv3 summary:
-    traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE, calibrated) or
-    valu_instructions_per_launch (SQ_INSTS_VALU)."""
-    if not PROFILE_SUMMARY.exists():
+def traffic_summary(lbm):
+    """The two-step fp32 kernel's calibrated HBM traffic and VALU count per launch, from the
+    summary the package's counter tool (counters.py: rocprofv3 FETCH_SIZE / WRITE_SIZE /
+    SQ_INSTS_VALU passes) wrote; `stale` when the device sources changed since."""
+    import importlib
+
+    counters = importlib.import_module(lbm.__name__ + ".counters")
+    s = counters.read_summary(TRAFFIC_SUMMARY)
+    if not s:
         return None
-    m = re.search(key + r":\s*([0-9.eE+]+)", PROFILE_SUMMARY.read_text())
-    return float(m.group(1)) if m else None
+    s["stale"] = s.get("source_tag") != counters.source_tag()
+    s["file"] = str(TRAFFIC_SUMMARY.relative_to(ROOT))
+    return s
 
 
 # Vector-ALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
@@ -285,19 +289,21 @@ def form_strip(lbm, args):
     import numpy as np
 
     nx, nyl = args.nx, args.ny // 8
-    out = {"shape": f"{nx}x({nyl}+2*ghost)", "timed_steps": 200,
-           "note": "one GPU's share of 8 strips, ghost rows refreshed on the device (one kernel) "
-                   "after every launch"}
+    out = {"shape": f"{nx}x({nyl}+2*ghost)", "timed_steps": 200, "runs": 3,
+           "note": "one GPU's share of 8 strips, ring closed on itself: one ghost row, ghost "
+                   "rows refreshed by a kernel after every step; two, each two-step launch "
+                   "stores its halo into its own next ghost rows; mlups = best of 3 runs"}
     for ghost in (1, 2):
         st = lbm.Strip(nx, nyl, "float", ghost)
         st.set_params(0.6, 1e-6, 0.0)
         st.upload_solid(np.zeros((nyl + 2 * ghost, nx), np.uint8))
-        st.timed_steps(20)
-        t = st.timed_steps(200)
-        mlups = nx * nyl * 200 / t / 1e6
+        st.timed_steps(1000)  # about 0.1 s of launches: the clocks settle after the upload
+        ts = [st.timed_steps(200) for _ in range(3)]
+        mlups = nx * nyl * 200 / min(ts) / 1e6
         st.close()
         per = 2 if ghost == 2 else 1
         out[f"ghost_{ghost}"] = {"mlups": round(mlups, 1), "steps_per_launch": per,
+                                 "mlups_runs": [round(nx * nyl * 200 / s / 1e6, 1) for s in ts],
                                  "halo_values_per_message": (3 if ghost == 1 else 12) * nx}
         log(f"strip {nx}x({nyl}+2x{ghost}): {mlups:.0f} MLUPS, {per} steps per launch")
     out["mlups"] = out["ghost_2"]["mlups"]
@@ -369,10 +375,15 @@ FORMS = {"single_step": form_single_step, "protocol": form_protocol,
          "cylinder": form_cylinder}
 
 
-def forms(lbm, args):
-    out = {}
-    wanted = list(FORMS) if args.forms in ("all", None) else args.forms.split(",")
-    for name in wanted:
+def wanted_forms(args):
+    return list(FORMS) if args.forms in ("all", None) else args.forms.split(",")
+
+
+def forms(lbm, args, out=None):
+    out = {} if out is None else out
+    for name in wanted_forms(args):
+        if name in out:
+            continue
         fn = FORMS[name]
         t0 = time.time()
         try:
@@ -408,6 +419,12 @@ def main():
 
     lbm = __graft_entry__.load_package()
     log(f"device: {lbm.device_info()}")
+    done = {}
+    if not args.no_forms and "copy_ceiling" in wanted_forms(args):
+        # measured first: about a second of HBM-bound copies, after which the clocks have left
+        # their idle state before the headline's warm-up steps start
+        done["copy_ceiling"] = form_copy_ceiling(lbm, args)
+        log(f"copy ceiling: {done['copy_ceiling']['copy_kernel_gbs']:.0f} GB/s")
     block = tuple(int(v) for v in args.block.split("x")) if args.block else None
     t, info, mass = run_plain(lbm, args.nx, args.ny, args.steps, args.warmup, "float", block)
     solver = json.loads("{" + info + "}")
@@ -441,20 +458,28 @@ def main():
                    "solver": solver},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": from_profile("traffic_bytes_per_launch"),
+                     "traffic": None,
                      "kernel": "step2_kernel<float>" if per_launch == 2 else "step_kernel<float>",
                      "updates_per_launch": per_launch,
                      "bytes_per_launch": BYTES_PER_CELL["float"] * cells,
                      "launch_ms": round(launch_s * 1e3, 5)},
     }
-    valu = from_profile("valu_instructions_per_launch")
+    ts = traffic_summary(lbm) if per_launch == 2 else None
+    valu = None
+    if ts:
+        line["roofline"]["traffic"] = ts["traffic_bytes_per_launch"]
+        line["roofline"]["traffic_source"] = {
+            "file": ts["file"], "stale": ts["stale"], "source_tag": ts["source_tag"],
+            "traffic_vs_algorithmic": ts["traffic_vs_algorithmic"],
+            "calibration": ts["calibration"]}
+        valu = ts.get("valu_instructions_per_launch")
     if valu and per_launch == 2:
         # the two-step kernel is bound by vector-ALU issue as much as by HBM: the profiled VALU
         # instruction count of a launch against the issue ceiling, over the launch time measured here
         line["roofline"]["valu_issue_frac"] = round(valu / VALU_WAVE_INSTR_PER_S / launch_s, 4)
         line["roofline"]["valu_instructions_per_launch"] = valu
     if not args.no_forms:
-        line["forms"] = forms(lbm, args)
+        line["forms"] = forms(lbm, args, done)
         f = line["forms"]
         cc = f.get("copy_ceiling", {})
         if cc.get("mlups_ceiling_fp32"):
