@@ -784,34 +784,90 @@ def equilibrium_field(ux: np.ndarray, uy: np.ndarray, rho: np.ndarray | float = 
     return g.astype(dtype).reshape(-1)
 
 
-def cylinder_instance(precision="float", nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0,
-                      Re=100.0, U=0.05, steps=40000, sample_every=20, window=0.5):
-    """Force-driven walled channel (periodic in x) with a staircase cylinder: tau and the body
-    force are set for Re = U D / nu with a Poiseuille profile of mean velocity U (peak 1.5 U),
-    which is also the initial state. Every `sample_every` steps two signals are recorded: the
-    transverse velocity u_y at the wake probe (xc + 3 D, yc), the signal the reference's
-    Strouhal script analyses (src/common/driver.hpp:175-214, scripts/strouhal.py:57-78), and
-    the lift (momentum exchange on the cylinder). The Strouhal number f D / U_w uses the mean
-    fluid velocity U_w over the analysis window (the last `window` fraction of the run).
-
-    The lift also carries the pressure pulse that the sudden obstacle sends round the periodic
-    channel (period about nx sqrt(3) steps, barely damped at this viscosity), so its zero
-    crossings are not a shedding measure; its FFT peak is reported as a cross-check.
-    Returns a dict with St from the probe (FFT and zero crossings, periods), St_fft of the lift,
-    U_w, Re_w, tau, force and both signals."""
+def _cylinder_setup(nx, ny, D, xc, yc, Re, U):
+    """tau, the mask, and the Poiseuille initial velocity of mean U of the cylinder instance."""
     nu = U * D / Re
     tau = 0.5 + 3.0 * nu
     H = ny - 2
     umax = 1.5 * U
-    force = 8.0 * nu * umax / (H * H)
     solid = np.maximum(make_channel(nx, ny), make_cylinder(nx, ny, xc, yc, D))
     yy = np.arange(ny, dtype=np.float64) - 0.5  # distance from the lower halfway wall
     prof = np.where((yy > 0) & (yy < H), 4 * umax * yy * (H - yy) / (H * H), 0.0)
     ux = np.repeat(prof[:, None], nx, axis=1).reshape(-1)
+    return nu, tau, solid, ux
+
+
+def _cylinder_solver(precision, nx, ny, tau, force, solid, ux):
     S = equilibrium_field(ux, np.zeros_like(ux), 1.0, real_dtype(precision))
     S.reshape(-1, 9)[solid.astype(bool)] = 0
     s = Solver(nx, ny, tau, force, 0.0, solid, precision)
     s.set_populations(S)
+    return s
+
+
+def cylinder_force(nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0, Re=100.0, U=0.05,
+                   steps=40000, sample_every=20, window=0.5, tol=2e-3, max_runs=8):
+    """The body force that holds the cylinder instance at Reynolds number Re: the mean fluid
+    velocity over the analysis window (the last `window` fraction of `steps`, sampled every
+    `sample_every` steps from the FP64 momentum reduction) equals U, so Re_w = U_w D / nu = Re
+    with nu = U D / Re. Found on the host by a secant iteration on FP64 runs of the whole
+    instance from its Poiseuille start, to a relative velocity error below `tol`. The force
+    that balances the walls alone (8 nu u_max / H^2 for an empty channel) leaves the flow
+    decelerating under the cylinder's drag (window mean 0.79 U at Re 100); the iteration
+    starts from four times it and a measured window-velocity response of 2.15e4 per unit force.
+    Returns (force, window velocity, runs as (force, U_w) pairs)."""
+    nu, tau, solid, ux = _cylinder_setup(nx, ny, D, xc, yc, Re, U)
+    fluid = float(nx * ny - int(solid.sum()))
+    k0 = int(steps // sample_every * (1 - window))
+
+    def window_velocity(force):
+        s = _cylinder_solver("double", nx, ny, tau, force, solid, ux)
+        s.run(k0 * sample_every)
+        u = []
+        for _ in range(steps // sample_every - k0):
+            s.run(sample_every)
+            u.append(s.momentum()[0] / fluid)
+        s.close()
+        return float(np.mean(u))
+
+    H = ny - 2
+    f = 4.0 * 8.0 * nu * 1.5 * U / (H * H)
+    slope = 2.15e4
+    runs = []
+    for _ in range(max_runs):
+        uw = window_velocity(f)
+        runs.append((f, uw))
+        if abs(uw / U - 1.0) < tol:
+            return f, uw, runs
+        if len(runs) >= 2 and runs[-1][0] != runs[-2][0]:
+            (f1, u1), (f2, u2) = runs[-2], runs[-1]
+            if u2 != u1:
+                slope = (u2 - u1) / (f2 - f1)
+        f = f + (U - uw) / slope
+    raise RuntimeError(f"cylinder force calibration did not converge: {runs}")
+
+
+def cylinder_instance(precision="float", nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0,
+                      Re=100.0, U=0.05, steps=40000, sample_every=20, window=0.5, force=None):
+    """Force-driven walled channel (periodic in x) with a staircase cylinder at Reynolds number
+    Re: tau from nu = U D / Re, the flow started from the Poiseuille profile of mean velocity U,
+    and the body force that keeps the mean fluid velocity over the analysis window at U
+    (cylinder_force(), computed once in FP64 unless `force` is given), so that
+    Re_w = U_w D / nu = Re. Every `sample_every` steps three signals are recorded: the lift on
+    the cylinder (momentum exchange, FP64), the transverse velocity u_y at the wake probe
+    (xc + 3 D, yc), the signal the reference's Strouhal script analyses
+    (src/common/driver.hpp:175-214, scripts/strouhal.py:57-78), and the mean fluid velocity
+    u_x (FP64 momentum / fluid cells). Over the analysis window (the last `window` fraction,
+    the reference script's default), the Strouhal number St = f D / U_w comes from the lift
+    (FFT peak and upward zero crossings), with the probe's as a cross-check; the velocity scale
+    U_w is the window mean of the fluid velocity.
+    Returns a dict with st_lift_fft, st_lift_zero, lift_periods, st_fft / st_zero / periods of
+    the probe, U_window, Re_window, tau, force, the calibration runs and the signals."""
+    nu, tau, solid, ux = _cylinder_setup(nx, ny, D, xc, yc, Re, U)
+    runs = None
+    if force is None:
+        force, _, runs = cylinder_force(nx, ny, D, xc, yc, Re, U, steps, sample_every, window)
+    s = _cylinder_solver(precision, nx, ny, tau, force, solid, ux)
     r = int(np.ceil(D / 2)) + 2
     box = (int(xc) - r, int(yc) - r, 2 * r + 1, 2 * r + 1)
     fluid = float(nx * ny - int(solid.sum()))
@@ -824,14 +880,15 @@ def cylinder_instance(precision="float", nx=1024, ny=256, D=32.0, xc=256.0, yc=1
         probe.append(float(cy @ f) / (1.0 + float(f.sum())))
         lift.append(s.momentum_exchange(*box)[1])
         umean.append(s.momentum()[0] / fluid)
+    s.close()
     lift, probe, umean = np.array(lift), np.array(probe), np.array(umean)
     k0 = int(len(lift) * (1 - window))
     Uw = float(umean[k0:].mean())
     st_fft, st_zero, periods = strouhal(probe[k0:], float(sample_every), D, Uw)
-    st_lift = strouhal(lift[k0:], float(sample_every), D, Uw)[0]
-    s.close()
-    return dict(st_fft=st_fft, st_zero=st_zero, periods=periods, st_lift_fft=st_lift,
-                U_window=Uw, Re_window=Uw * D / nu, tau=tau, force=force, probe=probe,
+    sl_fft, sl_zero, sl_periods = strouhal(lift[k0:], float(sample_every), D, Uw)
+    return dict(st_lift_fft=sl_fft, st_lift_zero=sl_zero, lift_periods=sl_periods,
+                st_fft=st_fft, st_zero=st_zero, periods=periods, U_window=Uw,
+                Re_window=Uw * D / nu, tau=tau, force=force, calibration=runs, probe=probe,
                 lift=lift, umean=umean)
 
 

@@ -15,7 +15,9 @@ Cases (reference file:line they come from):
   crop256_f64   256x256 walled channel, tau 0.6, gx 1e-6, 200 steps, rest init, double
                 (hash + sample only)
   cylinder_probe_1024x256  the force-driven periodic channel with a staircase cylinder
-                (D 32 at (256, 129), Re 100 at U 0.05, Poiseuille init, 40,000 steps, float):
+                (D 32 at (256, 129), Re 100 at U 0.05, Poiseuille init, 40,000 steps, float;
+                body force 6.498966063887593e-07, the value the package's cylinder_force()
+                computes on the GPU to hold the window-mean velocity at U):
                 u_y at the wake probe (xc + 3D, yc) every 20 steps, as the reference's
                 cylinder driver samples it (src/common/driver.hpp:175-214), and the final
                 populations' SHA-256. Written by the oracle's OpenMP pull solver (the reference's
@@ -78,7 +80,7 @@ def sample(a, k=257):
 
 
 CYL = dict(nx=1024, ny=256, D=32.0, xc=256.0, yc=129.0, Re=100.0, U=0.05, steps=40000,
-           every=20)
+           every=20, force=6.498966063887593e-07)
 
 
 def cylinder_probe_case():
@@ -89,7 +91,7 @@ def cylinder_probe_case():
     tau = 0.5 + 3.0 * nu
     H = ny - 2
     umax = 1.5 * c["U"]
-    force = 8.0 * nu * umax / (H * H)
+    force = c["force"]
     solid = np.maximum(oracle.make_channel(nx, ny), oracle.make_cylinder(nx, ny, c["xc"], c["yc"], D))
     yy = np.arange(ny, dtype=np.float64) - 0.5
     prof = np.where((yy > 0) & (yy < H), 4 * umax * yy * (H - yy) / (H * H), 0.0)
