@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "d2q9.hpp"
 
@@ -87,11 +88,26 @@ struct Stored<uint16_t> {
     const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, kAuxNone);
     return static_cast<float>(__builtin_bit_cast(_Float16, h));
   }
+  // the stored value itself, as a binary16 number (no conversion)
+  static __device__ __forceinline__ _Float16 load_h(Rsrc r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, kAuxNone));
+  }
   template <int kAux = kAuxNone>
   static __device__ __forceinline__ void store_bits(Rsrc r, unsigned vo, unsigned so,
                                                     unsigned short h) {
     __builtin_amdgcn_raw_buffer_store_b16(h, r, vo, so, kAux);
   }
+};
+
+// A stored value as the two-step walk carries it: float / double as computed, binary16
+// storage unconverted (the FP16 cell update converts where it reads).
+template <class S>
+struct WalkValue {
+  using type = typename Stored<S>::C;
+};
+template <>
+struct WalkValue<uint16_t> {
+  using type = _Float16;
 };
 
 // Nine plane resources of one population buffer (any size: each plane < 4 GiB).
@@ -105,6 +121,13 @@ struct Planes {
   template <class S>
   __device__ __forceinline__ typename Stored<S>::C load(int i, unsigned vo, unsigned so) const {
     return Stored<S>::load(r[i], vo, so);
+  }
+  template <class S>
+  __device__ __forceinline__ typename WalkValue<S>::type load_v(int i, unsigned vo, unsigned so) const {
+    if constexpr (std::is_same_v<S, uint16_t>)
+      return Stored<uint16_t>::load_h(r[i], vo, so);
+    else
+      return Stored<S>::load(r[i], vo, so);
   }
   template <class S, int kAux>
   __device__ __forceinline__ void store(int i, unsigned vo, unsigned so, typename Stored<S>::C v) const {
@@ -129,6 +152,13 @@ struct PlanesOne {
   template <class S>
   __device__ __forceinline__ typename Stored<S>::C load(int i, unsigned vo, unsigned so) const {
     return Stored<S>::load(r, vo, so + static_cast<unsigned>(i) * stride);
+  }
+  template <class S>
+  __device__ __forceinline__ typename WalkValue<S>::type load_v(int i, unsigned vo, unsigned so) const {
+    if constexpr (std::is_same_v<S, uint16_t>)
+      return Stored<uint16_t>::load_h(r, vo, so + static_cast<unsigned>(i) * stride);
+    else
+      return Stored<S>::load(r, vo, so + static_cast<unsigned>(i) * stride);
   }
   template <class S, int kAux>
   __device__ __forceinline__ void store(int i, unsigned vo, unsigned so, typename Stored<S>::C v) const {

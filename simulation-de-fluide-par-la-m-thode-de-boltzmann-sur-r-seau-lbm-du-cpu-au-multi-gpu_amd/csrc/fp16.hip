@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
         fast_collide(p.col, v);
       }
       unsigned short h[kQ];
-      round_cell(v, hash_cell(cell, p.seed), h);
+      round_cell(v, hash_cell(cell, p.seed), p.col.unit, h);
 #pragma unroll
       for (int i = 0; i < kQ; ++i) gn.store_bits(i, cols.c, rows.c, h[i]);
     }

@@ -38,8 +38,9 @@ __device__ __forceinline__ float half_value(unsigned short h) {
 }
 
 // Moment-compensated stochastic rounding of one cell's post-collision values (scaled units).
-// Rounding errors e = half(v) - v are exact in fp32 (v and its rounding are within a factor 2).
-__device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
+// Rounding errors e = half(v) - v are exact in fp32 (v and its rounding are within a factor 2);
+// each is one mixed-precision fma(half(v), one, -v) with one = 1 (see HalfCollide::unit).
+__device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r, float one,
                                            unsigned short (&h)[kQ]) {
   uint32_t rnd[kQ];
 #pragma unroll
@@ -48,7 +49,7 @@ __device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
 #pragma unroll
   for (int i = 1; i <= 4; ++i) {
     h[i] = to_half_sr(v[i], rnd[i]);
-    e[i] = half_value(h[i]) - v[i];
+    e[i] = __builtin_fmaf(half_value(h[i]), one, -v[i]);
   }
   const float ex = e[1] - e[3], ey = e[2] - e[4];
   float em = (e[1] + e[3]) + (e[2] + e[4]);
@@ -56,7 +57,7 @@ __device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
   for (int i = 5; i < kQ; ++i) {  // sum_d c_d (-(ex c_dx + ey c_dy) / 4) = -(ex, ey), sum_d = 0
     const float t = v[i] - 0.25f * (kCx[i] * ex + kCy[i] * ey);
     h[i] = to_half_sr(t, rnd[i]);
-    em += half_value(h[i]) - v[i];
+    em += __builtin_fmaf(half_value(h[i]), one, -v[i]);
   }
   h[0] = to_half_sr(v[0] - em, rnd[0]);
 }
@@ -64,38 +65,49 @@ __device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r,
 // Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
 __device__ __forceinline__ constexpr int wclass(int i) { return i == 0 ? 0 : (i < 5 ? 1 : 2); }
 
-// BGK + Guo in scaled units with the constants of HalfCollide. Every fused multiply-add is
-// written out (no contraction left to the compiler), so the one-step and two-step FP16 kernels
-// perform the same operations on a cell.
+// BGK + Guo in scaled units with the constants of HalfCollide: per pair of opposite directions
+// one even and one odd part (9 operations for two directions). The polynomial 4.5 cu^2 - 1.5 u.u
+// is formed per direction, so its two terms cancel at their own small magnitude. Every fused
+// multiply-add is written out (no contraction left to the compiler), so the one-step and
+// two-step FP16 kernels perform the same operations on a cell; sums with an operand that may
+// come straight from binary16 storage are fma(a, unit, b).
 __device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ]) {
-  const float m0 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])) + v[8];
-  const float d57 = v[5] - v[7], d86 = v[8] - v[6];
-  const float mx = ((v[1] - v[3]) + d57) + d86;
-  const float my = ((v[2] - v[4]) + d57) - d86;
+  const float one = c.unit;
+  auto add = [one](float a, float b) { return __builtin_fmaf(a, one, b); };
+  auto sub = [one](float a, float b) { return __builtin_fmaf(a, one, -b); };
+  const float m0 = add(v[8], (add(v[0], v[1]) + add(v[2], v[3])) + (add(v[4], v[5]) + add(v[6], v[7])));
+  const float d57 = sub(v[5], v[7]), d86 = sub(v[8], v[6]);
+  const float mx = (sub(v[1], v[3]) + d57) + d86;
+  const float my = (sub(v[2], v[4]) + d57) - d86;
   const float rho = __builtin_fmaf(m0, c.invS, 1.0f);
   const float ir = __builtin_amdgcn_rcpf(rho);
   const float ux = __builtin_fmaf(mx, c.invS, c.hfx) * ir;
   const float uy = __builtin_fmaf(my, c.invS, c.hfy) * ir;
   const float usq15 = 1.5f * __builtin_fmaf(ux, ux, uy * uy);
   const float uF = __builtin_fmaf(ux, c.fx, uy * c.fy);
-  float a[3], b[3], cuf[3];  // per class: omega w m0, omega w S rho, C_k u.F
+  float ac[3], b[3];  // per class: omega w S drho - C u.F, b_k
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    a[k] = c.ow[k] * m0;
+    ac[k] = __builtin_fmaf(c.ow[k], m0, -(c.C[k] * uF));
     b[k] = c.owS[k] * rho;
-    cuf[k] = c.C[k] * uF;
   }
-  // c_i . u per direction (0, ux, uy, -ux, -uy, ux + uy, uy - ux, -(ux + uy), ux - uy)
+  const float b3[3] = {0.0f, c.owS3[1] * rho, c.owS3[2] * rho};
+  // rest: cu = 0, A_0 = B_0 = 0
+  v[0] = __builtin_fmaf(c.om1, v[0], __builtin_fmaf(b[0], -usq15, ac[0]));
+  // pairs (direction, opposite) with the direction's cu: (1, 3) ux, (2, 4) uy, (5, 7) ux + uy,
+  // (6, 8) uy - ux
   const float spp = ux + uy, smp = uy - ux;
-  const float cus[kQ] = {0.0f, ux, uy, -ux, -uy, spp, smp, -spp, -smp};
+  constexpr int kFirst[4] = {1, 2, 5, 6};
+  const float cus[4] = {ux, uy, spp, smp};
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const int k = wclass(i);
-    const float cu = cus[i];
-    const float poly = __builtin_fmaf(__builtin_fmaf(4.5f, cu, 3.0f), cu, -usq15);
-    const float ogeq = __builtin_fmaf(b[k], poly, a[k]);
-    const float src = __builtin_fmaf(c.B[i], cu, c.A[i] - cuf[k]);
-    v[i] = __builtin_fmaf(c.om1, v[i], ogeq) + src;
+  for (int j = 0; j < 4; ++j) {
+    const int i = kFirst[j], o = kOpp[i], k = wclass(i);
+    const float cu = cus[j];
+    const float poly = __builtin_fmaf(4.5f * cu, cu, -usq15);
+    const float even = __builtin_fmaf(b[k], poly, __builtin_fmaf(c.B[i], cu, ac[k]));
+    const float odd = __builtin_fmaf(b3[k], cu, c.A[i]);
+    v[i] = __builtin_fmaf(c.om1, v[i], even + odd);
+    v[o] = __builtin_fmaf(c.om1, v[o], even - odd);
   }
 }
 

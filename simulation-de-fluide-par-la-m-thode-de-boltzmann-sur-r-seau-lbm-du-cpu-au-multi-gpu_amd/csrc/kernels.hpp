@@ -45,12 +45,20 @@ struct StepParams {
 };
 
 // Collision constants of the FP16-storage step (scaled units g' = S g), computed once per launch
-// on the host (and per cell on the device for a per-cell force field):
-//   omega geq_i = b_k poly_i + a_k with a_k = omega w_k m0', b_k = omega w_k S rho (class k),
-//   Guo source src'_i = B_i c_i.u + A_i - C_k u.F.
+// on the host (and per cell on the device for a per-cell force field). BGK towards the shifted
+// equilibrium plus the Guo source; with b_k = omega w_k S rho for the weight class k of a
+// direction (rest, axis, diagonal), cu = c_i . u and S drho = sum g':
+//   omega geq'_i = b_k (4.5 cu^2 - 1.5 u.u) + 3 b_k cu + omega w_k S drho,
+//   fp src'_i   = A_i + B_i cu - C_k u.F,  A_i = 3 S fp w_i c_i.F, B_i = 9 S fp w_i c_i.F,
+//                                           C_k = 3 S fp w_k.
+// A direction and its opposite (cu and -cu) share the even part and differ in the sign of the
+// odd part 3 b_k cu + A_i. `unit` is 1: sums are written as fma(a, unit, b), exactly a + b, so
+// that an operand converted from a stored binary16 value is read in its 16-bit form by a
+// mixed-precision fused multiply-add instead of being converted by an instruction of its own
+// (the compiler turns a literal fma(a, 1, b) back into an addition).
 struct HalfCollide {
-  float om1, invS, hfx, hfy, fx, fy;
-  float ow[3], owS[3];  // omega w_k, omega w_k S for the rest, axis and diagonal classes
+  float om1, invS, hfx, hfy, fx, fy, unit;
+  float ow[3], owS[3], owS3[3];  // omega w_k, omega w_k S, 3 omega w_k S
   float A[9], B[9], C[3];
 
   __host__ __device__ void setup(float omega, float fp, float S, float inv_s, float fx_, float fy_) {
@@ -60,16 +68,18 @@ struct HalfCollide {
     fy = fy_;
     hfx = 0.5f * fx_;
     hfy = 0.5f * fy_;
+    unit = 1.0f;
     for (int i = 0; i < kQ; ++i) {
-      const float w = weight<float>(i);
+      const float wi = weight<float>(i);
       const float cf = kCx[i] * fx_ + kCy[i] * fy_;
-      A[i] = 3.0f * S * fp * w * cf;
-      B[i] = 9.0f * S * fp * w * cf;
+      A[i] = 3.0f * S * fp * wi * cf;
+      B[i] = 9.0f * S * fp * wi * cf;
     }
     for (int k = 0; k < 3; ++k) {
       const float w = weight<float>(k == 0 ? 0 : (k == 1 ? 1 : 5));
       ow[k] = omega * w;
       owS[k] = omega * w * S;
+      owS3[k] = 3.0f * omega * w * S;
       C[k] = 3.0f * S * fp * w;
     }
   }
@@ -201,7 +211,8 @@ int step2_tile_rows(const StepParams<T>& p, int block_x, int min_waves = 1);
 
 // FP16 storage, two steps per launch (step2.hip): the conditions of launch_step2; p.seed is the
 // stochastic-rounding seed of the first step, seed2 of the second. Bit-identical to two
-// launch_step_fp16 steps. LDS per workgroup: step2_lds_bytes(block_x, sizeof(float)).
+// launch_step_fp16 steps. LDS per workgroup: step2_lds_bytes(block_x, 2) (the ring holds the
+// step-1 values in binary16).
 hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream);
 int step2_tile_rows_fp16(const StepParamsH& p, int block_x);
 

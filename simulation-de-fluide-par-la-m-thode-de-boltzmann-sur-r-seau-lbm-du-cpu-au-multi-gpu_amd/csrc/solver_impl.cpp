@@ -186,9 +186,11 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
     xcd_bands = prec == LBM_DOUBLE ? 1 : 0;
   }
   // two-step kernel: 512-column tiles in float; 256 in double (its LDS ring doubles; 512
-  // columns exceed the LDS of a workgroup). 8192^2: fp32 512 x 64 146.3k MLUPS (256 x 64
-  // 128.5k), fp64 256 x 64 69.6k (128 x 64 63.7k); the height comes from step2_tile_rows().
-  block2_x = prec == LBM_DOUBLE ? 256 : 512;
+  // columns exceed the LDS of a workgroup) and in FP16 storage (87 VGPRs: five 4-wave
+  // workgroups per CU). 8192^2: fp32 512 x 64 146.3k MLUPS (256 x 64 128.5k), fp64 256 x 64
+  // 69.6k (128 x 64 63.7k), FP16 256 x 64 210.6k (512 x 64 202.9k); the height comes from
+  // step2_tile_rows().
+  block2_x = prec == LBM_FLOAT ? 512 : 256;
   rows2 = -1;
   LBM_TRY(parse_block_env(&block_x, &rows));
   LBM_HIP_TRY(hipGetDevice(&device));

@@ -111,11 +111,11 @@ struct Step2View {
 template <class T>
 struct PlainCells {
   using S = T;
-  using C = T;
+  using Ring = T;  // the values the walk carries (registers and LDS)
   using Word = T;
   T omega, fp, fx, fy;
-  __device__ __forceinline__ void step1(C (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
-  __device__ __forceinline__ void step2(C (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
+  __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
+  __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
     collide(v, fx, fy, omega, fp);
 #pragma unroll
     for (int i = 0; i < kQ; ++i) out[i] = v[i];
@@ -129,22 +129,31 @@ struct PlainCells {
 // FP16 storage: the collision and the stochastic rounding of step_fp16_kernel, with the seed of
 // each of the two steps, so the result is bit-identical to two one-step launches; step 2 reads
 // the rounded step-1 values, as the second launch would.
+// The walk carries binary16 numbers (the stored values and the rounded step-1 values), in
+// registers and in the LDS ring; they are converted only inside the cell update, where every
+// use is a (mixed-precision) fused multiply-add that reads the 16-bit value directly.
 struct HalfCells {
   using S = uint16_t;
-  using C = float;
+  using Ring = _Float16;
   using Word = unsigned short;
   HalfCollide col;
   uint32_t seed1, seed2;
-  __device__ __forceinline__ void step1(C (&v)[kQ], uint32_t cell) const {
-    fast_collide(col, v);
-    unsigned short h[kQ];
-    round_cell(v, hash_cell(cell, seed1), h);
+  __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t cell) const {
+    float f[kQ];
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) v[i] = half_value(h[i]);
+    for (int i = 0; i < kQ; ++i) f[i] = static_cast<float>(v[i]);
+    fast_collide(col, f);
+    unsigned short h[kQ];
+    round_cell(f, hash_cell(cell, seed1), col.unit, h);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) v[i] = __builtin_bit_cast(_Float16, h[i]);
   }
-  __device__ __forceinline__ void step2(C (&v)[kQ], uint32_t cell, Word (&out)[kQ]) const {
-    fast_collide(col, v);
-    round_cell(v, hash_cell(cell, seed2), out);
+  __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t cell, Word (&out)[kQ]) const {
+    float f[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = static_cast<float>(v[i]);
+    fast_collide(col, f);
+    round_cell(f, hash_cell(cell, seed2), col.unit, out);
   }
   template <int, class PS>
   __device__ __forceinline__ static void store(const PS& gn, int i, unsigned vo, unsigned so, Word w) {
@@ -158,20 +167,20 @@ struct HalfCells {
 // exactly as update_column.
 template <class P, class PS>
 __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, const P& cells,
-                                          const PS& g, int c, int r, typename P::C (&v)[kQ]) {
+                                          const PS& g, int c, int r, typename P::Ring (&v)[kQ]) {
   using S = typename P::S;
   constexpr unsigned B = sizeof(S);
   const Rows rows = rows_of(r, p.nx, p.ny, B);
   const Cols cols = cols_of(c, p.nx, B);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) v[i] = g.template load<S>(i, cols.of(kCx[i]) + rows.of(kCy[i]), 0u);
+  for (int i = 0; i < kQ; ++i) v[i] = g.template load_v<S>(i, cols.of(kCx[i]) + rows.of(kCy[i]), 0u);
   uint32_t link = 0;
   if (p.segflags[static_cast<size_t>(r) * p.nseg + (c >> 6)]) {
     link = p.links[static_cast<size_t>(r) * p.nx + c];
     if (!(link & kLinkSolid)) {
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
-        if (link & (1u << i)) v[i] = g.template load<S>(kOpp[i], cols.c + rows.c, 0u);
+        if (link & (1u << i)) v[i] = g.template load_v<S>(kOpp[i], cols.c + rows.c, 0u);
     }
   }
   if (!(link & kLinkSolid))
@@ -197,7 +206,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
                                            const int x0, const int y0, const int H,
                                            const int row_end, unsigned char* smem,
                                            const HaloOut<typename P::S>* hout = nullptr) {
-  using T = typename P::C;  // registers and LDS hold computed values
+  using T = typename P::Ring;  // the values the walk carries, in registers and in LDS
   using S = typename P::S;
   const int W = static_cast<int>(blockDim.x), WP = W + 2;
   T* ring = reinterpret_cast<T*>(smem);                  // [kRingSlots][kRingPlanes][W + 2]
@@ -248,7 +257,11 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
 #pragma unroll
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
   T cur[kQ];            // raw pulls of the current step-1 row
-  pull_streamed<S>(g, uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cols, cur);
+  auto pull = [&](const Rows& rw, T (&out)[kQ]) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) out[i] = g.template load_v<S>(i, cols.of(kCx[i]), rw.of(kCy[i]));
+  };
+  pull(uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cur);
 
   // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
   // are issued first and nothing between them and their use branches around a vector memory
@@ -260,7 +273,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     const int r = wrap_y(y0 - 1 + k);
     const Rows rows = uniform_rows_of(r, nx, ny, B);
     T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
-    pull_streamed<S>(g, uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), cols, nxt);
+    pull(uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), nxt);
     // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
     // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
@@ -271,7 +284,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       link = p.links[static_cast<size_t>(r) * nx + c];
       T own[kQ];
 #pragma unroll
-      for (int i = 1; i < kQ; ++i) own[i] = g.template load<S>(kOpp[i], cols.c, rows.c);
+      for (int i = 1; i < kQ; ++i) own[i] = g.template load_v<S>(kOpp[i], cols.c, rows.c);
 #pragma unroll
       for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? own[i] : v[i];
     }
@@ -556,9 +569,12 @@ int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds, int min_wav
   // workgroups at least 97 % full), else the best-filling one. 8192^2 fp32, 512 columns, two
   // workgroups per CU: 64 rows (2048 tiles, four full waves) 146.3k MLUPS; 72 rows (1824 tiles,
   // last wave 56 % full) 125.8k, 56 rows 128.3k; 32 rows (eight waves, more recomputed rows)
-  // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k. min_waves: candidates with
-  // fewer full waves are passed over unless none has that many (a strip's 8192 x 1024 owned
-  // rows: 16 rows, two waves, 136.2k MLUPS against 133.7k for 32 rows, one wave).
+  // 141.9k; 128 and 256 rows (two waves, one) 134.9k and 132.9k. With three waves or more a
+  // partly filled last wave costs little (workgroups finish at different times): FP16 storage,
+  // 256 columns, five workgroups per CU: 64 rows (3.2 waves) 210.6k against 205.8k for the
+  // 16 rows that fill the slots evenly. min_waves: candidates with fewer full waves are passed
+  // over unless none has that many (a strip's 8192 x 1024 owned rows: 16 rows, two waves,
+  // 136.2k MLUPS against 133.7k for 32 rows, one wave).
   const long slots = static_cast<long>(per_cu) * cus;
   const long tiles_x = (nx + block_x - 1) / block_x;
   int best = 16;
@@ -566,7 +582,8 @@ int tile_rows_for(K kernel, int nx, int ny, int block_x, size_t lds, int min_wav
   for (const int rows : {64, 48, 32, 24, 16}) {
     const long tiles = tiles_x * ((ny + rows - 1) / rows);
     const double fill = static_cast<double>(tiles) / (((tiles + slots - 1) / slots) * slots);
-    if (fill >= 0.97 && (tiles >= min_waves * slots || rows == 16)) return rows;
+    if ((fill >= 0.97 || tiles >= 3 * slots) && (tiles >= min_waves * slots || rows == 16))
+      return rows;
     if (fill > best_fill) {
       best_fill = fill;
       best = rows;
@@ -602,7 +619,7 @@ hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, 
       static_cast<uint64_t>(p.N) * sizeof(uint16_t) >= (uint64_t{1} << 32) ||
       p.N < static_cast<size_t>(p.nx) * p.ny + kPlaneTail)
     return hipErrorInvalidValue;
-  const size_t lds = step2_lds_bytes(block_x, sizeof(float));
+  const size_t lds = step2_lds_bytes(block_x, sizeof(_Float16));
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const unsigned tiles_y = static_cast<unsigned>((p.ny + p.rows_per_tile - 1) / p.rows_per_tile);
   if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
@@ -615,7 +632,7 @@ hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, 
 }
 
 int step2_tile_rows_fp16(const StepParamsH& p, int block_x) {
-  const size_t lds = step2_lds_bytes(block_x, sizeof(float));
+  const size_t lds = step2_lds_bytes(block_x, sizeof(_Float16));
   return fp16_one_resource(p) ? tile_rows_for(step2_fp16_kernel<PlanesOne>, p.nx, p.ny, block_x, lds)
                               : tile_rows_for(step2_fp16_kernel<Planes>, p.nx, p.ny, block_x, lds);
 }
