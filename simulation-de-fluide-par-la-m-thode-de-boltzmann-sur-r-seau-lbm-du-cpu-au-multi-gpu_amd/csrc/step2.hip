@@ -113,6 +113,9 @@ struct PlainCells {
   using S = T;
   using Ring = T;  // the values the walk carries (registers and LDS)
   using Word = T;
+  // step 2's bounce-back reads the step-1 row above from the LDS ring (6 fewer values in
+  // registers: fp32 117 -> 111 VGPRs, fp64 195 -> 183)
+  static constexpr bool kRowInRegisters = false;
   T omega, fp, fx, fy;
   __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
   __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
@@ -136,6 +139,9 @@ struct HalfCells {
   using S = uint16_t;
   using Ring = _Float16;
   using Word = unsigned short;
+  // the FP16 walk keeps the whole step-1 row in registers (87 VGPRs; reading it from the ring
+  // measured 109)
+  static constexpr bool kRowInRegisters = true;
   HalfCollide col;
   uint32_t seed1, seed2;
   __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t cell) const {
@@ -251,11 +257,13 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   // last cell (a per-lane offset instead of a branch around the stores, see below).
   const unsigned tail = static_cast<unsigned>(nx * ny) * B + static_cast<unsigned>(t & 63) * B;
   const Cols cols = cols_of(c, nx, B);
-  T prev[kQ];           // R_{t+1} at (c, r - 1)
-  T prev2 = T(0);       // plane 2 of R_{t+1} at (c, r - 2)
-  uint32_t prev_link = kLinkSolid;
+  // R_{t+1} at (c, r - 1): all planes (kRowInRegisters), or those with c_x = 0 (the others are
+  // in the ring slot of row r - 1, where the bounce-back of rows next to a wall reads them)
+  T prev[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
+  T prev2 = T(0);       // plane 2 of R_{t+1} at (c, r - 2)
+  uint32_t prev_link = kLinkSolid;
   T cur[kQ];            // raw pulls of the current step-1 row
   auto pull = [&](const Rows& rw, T (&out)[kQ]) {
 #pragma unroll
@@ -321,9 +329,17 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       w[8] = s0[ring_index(8) * WP + tt];
       const uint32_t bb = prev_link & kLinkBounce;
       if (__builtin_amdgcn_ballot_w64(bb != 0)) {  // wave-uniform: only rows next to a wall
+        // R_{t+1} at (c, q)
+        T pq[kQ];
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) pq[i] = prev[i];
+        if constexpr (!P::kRowInRegisters) {
+#pragma unroll
+          for (int i : {1, 3, 5, 6, 7, 8}) pq[i] = s1[ring_index(i) * WP + tt + 1];
+        }
 #pragma unroll
         for (int i = 1; i < kQ; ++i)
-          if (bb & (1u << i)) w[i] = prev[kOpp[i]];
+          if (bb & (1u << i)) w[i] = pq[kOpp[i]];
       }
       typename P::Word out[kQ];
       const int q = y0 - 2 + k;
@@ -369,7 +385,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     prev2 = prev[2];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
-      prev[i] = v[i];
+      if (P::kRowInRegisters || kCx[i] == 0) prev[i] = v[i];
       cur[i] = nxt[i];
     }
     prev_link = link;
