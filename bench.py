@@ -260,9 +260,15 @@ def form_precisions(lbm, args):
 def form_batched(lbm, args):
     doms = lbm.random_domains(seed=0, n=256)
     b = lbm.Batch(doms, "float")
-    tx, tr, wg, tuned = b.autotune(steps=3)
+    if args.batch_tile:  # a fixed two-step shape instead of the autotuner (profiling runs)
+        btx, bty = (int(v) for v in args.batch_tile.split("x"))
+        b.set_steps_per_launch(2, btx, bty, 0)
+        tx = tr = wg = None  # the one-step shape is not used
+    else:
+        tx, tr, wg, tuned = b.autotune(steps=3)
     order, nt = b.order()
     two = b.steps_per_launch()
+    layout = b.two_step_layout()
     b.set_graph_steps(100)
     b.run(100)
     t = b.timed_run(1000)
@@ -271,11 +277,18 @@ def form_batched(lbm, args):
     b.close()
     per_launch = two["steps_per_launch"]
     log(f"batched: {mlups:.0f} MLUPS over {cells} cells ({per_launch} steps per launch, two-step "
-        f"tile {two['tile_x']}x{two['tile_rows']}; one-step order {order}, tile {tx}x{tr}, {wg} WGs)")
+        f"sub-tiles {two['tile_x']}x{two['tile_rows']} in {layout['items']} items, fill "
+        f"{layout['fill']:.3f}; "
+        f"one-step order {order}, tile {tx}x{tr}, {wg} WGs)")
     return {"domains": 256, "seed": 0, "cells": cells, "mlups": round(mlups, 1),
             "timed_steps": 1000, "graph_steps": 100, "steps_per_launch": per_launch,
-            "two_step_tile": f"{two['tile_x']}x{two['tile_rows']}",
+            "two_step_subtile": f"{two['tile_x']}x{two['tile_rows']}",
+            "two_step_band_rows_modelled": layout["modelled"],
+            "two_step_items": layout["items"], "two_step_threads": layout["threads"],
+            "two_step_fill": round(layout["fill"], 4),
             "two_step_workgroups": two["workgroups"],
+            "two_step_grid": "one workgroup per work item" if two["workgroups"] == 0
+            else "persistent",
             "one_step": {"order": order, "streaming_stores": nt, "tile_x": tx, "tile_rows": tr,
                          "workgroups": wg},
             "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / per_launch / 1e9, 1)}
@@ -421,6 +434,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the other forms")
     ap.add_argument("--forms", default="all",
                     help="comma-separated subset of: " + ",".join(FORMS) + " (default all)")
+    ap.add_argument("--batch-tile", default=None,
+                    help="batched form: the two-step maximum tile WxH instead of autotuning")
     ap.add_argument("--cpu-full", action="store_true",
                     help="CPU baseline: the full protocol steps at one thread too (slow)")
     args = ap.parse_args()

@@ -264,14 +264,23 @@ int lbm_batch_set_order(lbm_batch* b, int order, int streaming_stores);
 int lbm_batch_get_order(const lbm_batch* b, int* order, int* streaming_stores);
 /* Temporal blocking over the batch (every domain is periodic with a uniform force): with
  * steps_per_launch = 2 (the default) lbm_batch_run / lbm_batch_timed_run and their graphs
- * advance pairs of steps in one launch of the two-step kernel over a list of tiles of tile_x
- * columns (64..512, multiple of 64; double at most 256) by tile_rows rows (0 keeps either),
- * walked by `workgroups` persistent workgroups (0, the default: one per tile; -1 keeps); an
- * odd count ends with one single step. Same bits as single steps. */
+ * advance pairs of steps in one launch of the two-step kernel. Its work: each domain's columns
+ * in sub-tiles of tile_x columns (a multiple of 64 up to the kernel's 512 threads, double 256;
+ * default: that maximum) plus one of the remainder, its rows in bands of about tile_rows rows
+ * (-1, the default: the height the built-in schedule model picks for these domains; 0 keeps);
+ * the sub-tiles are packed into work items of one workgroup each. `workgroups`: 0 (the
+ * default) one workgroup per item; G > 0 a persistent grid of G; -1 keeps. An odd count ends
+ * with one single step. Same bits as single steps. */
 int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_x, int tile_rows,
                                    int workgroups);
+/* tile_rows: the band height in use (modelled or set). */
 int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, int* tile_x,
                                    int* tile_rows, int* workgroups);
+/* The two-step work list: threads per workgroup, work items, band rows in use, whether they are
+ * the modelled choice, and the fill (cell rows over lane rows of the items, 0..1). Any output
+ * may be null. */
+int lbm_batch_two_step_layout(const lbm_batch* b, int* threads, int* items, int* band_rows,
+                              int* modelled, double* fill);
 int lbm_batch_run(lbm_batch* b, int steps);
 int lbm_batch_timed_run(lbm_batch* b, int steps, double* seconds);
 int lbm_batch_populations(lbm_batch* b, int domain, void* out);

@@ -127,6 +127,7 @@ def _declare(L):
         "lbm_batch_get_order": ([vp, ip, ip], i),
         "lbm_batch_set_steps_per_launch": ([vp, i, i, i, i], i),
         "lbm_batch_get_steps_per_launch": ([vp, ip, ip, ip, ip], i),
+        "lbm_batch_two_step_layout": ([vp, ip, ip, ip, ip, dp], i),
         "lbm_adjoint_create": ([i, i, d, _u8p, i, i, i, i, i, i, i, i, C.POINTER(vp)], i),
         "lbm_adjoint_destroy": ([vp], i),
         "lbm_adjoint_set_force_field": ([vp, vp], i),
@@ -654,8 +655,9 @@ class Batch:
     def set_steps_per_launch(self, steps_per_launch: int, tile_x: int = 0, tile_rows: int = 0,
                              workgroups: int = -1):
         """1: one step per launch; 2 (default): pairs of steps in one two-step launch (same
-        bits); tile_x / tile_rows set its tiles (0 keeps); workgroups its persistent grid (0:
-        one workgroup per tile, -1 keeps)."""
+        bits); tile_x: sub-tile width (0 keeps); tile_rows: band height (0 keeps, -1 the
+        modelled height); workgroups its persistent grid (0: one workgroup per work item, -1
+        keeps)."""
         check(lib().lbm_batch_set_steps_per_launch(self._h, int(steps_per_launch), int(tile_x),
                                                    int(tile_rows), int(workgroups)))
 
@@ -665,6 +667,16 @@ class Batch:
                                                    C.byref(w)))
         return {"steps_per_launch": a.value, "tile_x": b.value, "tile_rows": c.value,
                 "workgroups": w.value}
+
+    def two_step_layout(self) -> dict:
+        """The two-step work list: threads per workgroup, work items, band rows in use, whether
+        they are the modelled choice, and the fill (cell rows over lane rows of the items)."""
+        th, it, rows, mod = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        fill = C.c_double()
+        check(lib().lbm_batch_two_step_layout(self._h, C.byref(th), C.byref(it), C.byref(rows),
+                                              C.byref(mod), C.byref(fill)))
+        return {"threads": th.value, "items": it.value, "band_rows": rows.value,
+                "modelled": bool(mod.value), "fill": fill.value}
 
     def autotune(self, steps=10):
         tx, tr, wg, m = C.c_int(), C.c_int(), C.c_int(), C.c_double()

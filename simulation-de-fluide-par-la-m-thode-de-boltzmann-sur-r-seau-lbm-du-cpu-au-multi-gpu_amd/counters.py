@@ -86,11 +86,12 @@ def read_counters(path: str | os.PathLike) -> list[dict]:
     return sorted(acc.values(), key=lambda e: e["dispatch"])
 
 
-def per_dispatch(rows: list[dict], kernel: str, counter: str, skip: int = 0) -> list[float]:
+def per_dispatch(rows: list[dict], kernel: str, counter: str, skip: int = 0,
+                 last: int = 0) -> list[float]:
     """Values of `counter` for the dispatches whose kernel name contains `kernel`, after
-    leaving out the first `skip` of them."""
-    vals = [e["value"] for e in rows if kernel in e["kernel"] and e["counter"] == counter]
-    return vals[skip:]
+    leaving out the first `skip` of them; with last > 0 only the last `last` of those."""
+    vals = [e["value"] for e in rows if kernel in e["kernel"] and e["counter"] == counter][skip:]
+    return vals[-last:] if last > 0 else vals
 
 
 def _mean(v: list[float]) -> float:
@@ -100,16 +101,22 @@ def _mean(v: list[float]) -> float:
 
 
 def summarize(fetch_rows, write_rows, kernel: str, cells: int, updates: int,
-              bytes_per_cell: float, valu_rows=None, skip: int = 0,
+              bytes_per_cell: float, valu_rows=None, skip: int = 0, last: int = 0,
+              dispatches_per_launch: int = 1,
               calibration_kernel: str = CALIBRATION_KERNEL,
               calibration_bytes: int = CALIBRATION_BYTES) -> dict:
-    """Calibrated HBM bytes per launch of `kernel` (mean over its dispatches)."""
+    """Calibrated HBM bytes per launch of `kernel`: the mean over its selected dispatches times
+    dispatches_per_launch (a launch of the batched form is one dispatch per tile-width class;
+    `last` then selects whole launches)."""
+    if last and last % dispatches_per_launch:
+        raise ValueError("last must be a whole number of launches")
     f_cal = calibration_bytes / 1024 / _mean(per_dispatch(fetch_rows, calibration_kernel, "FETCH_SIZE"))
     w_cal = calibration_bytes / 1024 / _mean(per_dispatch(write_rows, calibration_kernel, "WRITE_SIZE"))
-    f = per_dispatch(fetch_rows, kernel, "FETCH_SIZE", skip)
-    w = per_dispatch(write_rows, kernel, "WRITE_SIZE", skip)
-    read = _mean(f) * f_cal * 1024
-    write = _mean(w) * w_cal * 1024
+    f = per_dispatch(fetch_rows, kernel, "FETCH_SIZE", skip, last)
+    w = per_dispatch(write_rows, kernel, "WRITE_SIZE", skip, last)
+    k = dispatches_per_launch
+    read = _mean(f) * k * f_cal * 1024
+    write = _mean(w) * k * w_cal * 1024
     algorithmic = bytes_per_cell * cells
     out = {"kernel": kernel, "dispatches": {"fetch": len(f), "write": len(w)},
            "fetch_kib_per_dispatch": round(_mean(f), 1), "write_kib_per_dispatch": round(_mean(w), 1),
@@ -118,16 +125,16 @@ def summarize(fetch_rows, write_rows, kernel: str, cells: int, updates: int,
            "read_bytes_per_launch": round(read), "write_bytes_per_launch": round(write),
            "traffic_bytes_per_launch": round(read + write),
            "algorithmic_bytes": round(algorithmic), "cells": cells,
-           "updates_per_launch": updates,
+           "updates_per_launch": updates, "dispatches_per_launch": k,
            "traffic_vs_algorithmic": round((read + write) / algorithmic, 4),
            "read_vs_algorithmic": round(read / (algorithmic / 2), 4),
            "bytes_per_cell_update": round((read + write) / (cells * updates), 2),
            "source_tag": source_tag()}
     if valu_rows is not None:
-        v = per_dispatch(valu_rows, kernel, "SQ_INSTS_VALU", skip)
+        v = per_dispatch(valu_rows, kernel, "SQ_INSTS_VALU", skip, last)
         if v:
-            out["valu_instructions_per_launch"] = round(_mean(v))
-            out["valu_instructions_per_update"] = round(_mean(v) * 64 / (cells * updates), 1)
+            out["valu_instructions_per_launch"] = round(_mean(v) * k)
+            out["valu_instructions_per_update"] = round(_mean(v) * k * 64 / (cells * updates), 1)
     return out
 
 
@@ -139,6 +146,7 @@ def write_summary(path: str | os.PathLike, s: dict, commands: list[str]) -> None
     md += ["", "| quantity | value |", "|---|---|",
            f"| dispatches of the kernel (fetch / write pass) | {s['dispatches']['fetch']} / "
            f"{s['dispatches']['write']} |",
+           f"| dispatches per launch | {s.get('dispatches_per_launch', 1)} |",
            f"| FETCH_SIZE KiB per dispatch | {s['fetch_kib_per_dispatch']:,} |",
            f"| WRITE_SIZE KiB per dispatch | {s['write_kib_per_dispatch']:,} |",
            f"| calibration ({s['calibration']['kernel']}, {s['calibration']['bytes']:,} B each "
@@ -194,6 +202,10 @@ def main(argv=None) -> None:
         sp.add_argument("--bytes-per-cell", type=float, required=True,
                         help="algorithmic bytes per cell per launch")
         sp.add_argument("--skip", type=int, default=0, help="dispatches left out at the start")
+        sp.add_argument("--last", type=int, default=0,
+                        help="only the last N dispatches (after --skip)")
+        sp.add_argument("--dispatches-per-launch", type=int, default=1,
+                        help="dispatches of the kernel that make one launch of the path")
         sp.add_argument("--out", required=True, help="summary Markdown file")
     r = sub.choices["run"]
     r.add_argument("--work", default="gpurun_out/counters", help="rocprofv3 output directory")
@@ -221,12 +233,14 @@ def main(argv=None) -> None:
             commands.append(vl)
             vrows = read_counters(vcsv)
         summary = summarize(read_counters(fcsv), read_counters(wcsv), a.kernel, a.cells,
-                            a.updates_per_launch, a.bytes_per_cell, vrows, a.skip)
+                            a.updates_per_launch, a.bytes_per_cell, vrows, a.skip, a.last,
+                            a.dispatches_per_launch)
     else:
         commands = [made_by]
         summary = summarize(read_counters(a.fetch), read_counters(a.write), a.kernel, a.cells,
                             a.updates_per_launch, a.bytes_per_cell,
-                            read_counters(a.valu_csv) if a.valu_csv else None, a.skip)
+                            read_counters(a.valu_csv) if a.valu_csv else None, a.skip, a.last,
+                            a.dispatches_per_launch)
     write_summary(a.out, summary, commands)
     print(json.dumps(summary), flush=True)
 

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -48,13 +50,17 @@ struct lbm_batch {
   size_t segs_cap = 0;
   int ntiles = 0;             // entries of the list the current order walks
   int tile_x = 256, tile_rows = 4, workgroups = 0;
-  // two steps per launch: its own tile list (tile2_x columns by tile2_rows rows)
+  // Two steps per launch: per-domain sub-tiles of at most tile2_x columns and band height
+  // rows2 (tile2_rows, or the modelled choice when it is 0), packed into work items of block2
+  // threads (build_items2).
   int steps_per_launch = 2;
-  int tile2_x = 256, tile2_rows = 32;
-  int workgroups2 = 0;  // persistent grid of the two-step kernel (0: one workgroup per tile)
-  BatchTile* dtiles2 = nullptr;
-  size_t tiles2_cap = 0;
-  int ntiles2 = 0;
+  int block2 = 512;  // threads per workgroup of the two-step kernel (double: 256)
+  int tile2_x = 512, tile2_rows = 0, rows2 = 0;
+  int workgroups2 = 0;  // persistent grid of the two-step kernel (0: one workgroup per item)
+  BatchWave* dwaves = nullptr;
+  size_t waves_cap = 0;
+  int nitems2 = 0;
+  double fill2 = 0.0;   // cell rows / lane rows of the items
   int order = 0, store_nt = 1;  // tile order (see batch.hip) and non-temporal stores
   bool one_rsrc = true;         // every domain's nine planes span < 4 GiB
   int graph_steps = 100;
@@ -80,7 +86,7 @@ struct lbm_batch {
     drop_graphs();
     for (void* p : {g, gn, static_cast<void*>(links), static_cast<void*>(segflags), ddesc,
                     static_cast<void*>(dtiles), static_cast<void*>(dsegs),
-                    static_cast<void*>(dtiles2), staging})
+                    static_cast<void*>(dwaves), staging})
       if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -153,47 +159,154 @@ struct lbm_batch {
     return LBM_OK;
   }
 
-  // Two-step work-list: tiles of tile2_x columns by tile2_rows rows, domain by domain, row by
-  // row, so that the tiles running together are neighbours sharing halo rows and edge lines in
-  // L2 (a largest-first order measured 14-17 % slower on the 256 seeded domains).
-  int build_tiles2() {
-    std::vector<BatchTile> t;
+  // ---- two-step work items ----
+  struct Sub {
+    int domain, x0, y0, rows, width;
+  };
+  struct Item {
+    int nr_max, free;
+    std::vector<int> subs;
+  };
+
+  // Sub-tiles: domain k's columns in tiles of tile2_x columns and one last tile of the remainder
+  // (its width rounded up to 64: the remainders of different domains pack together), its rows
+  // in ceil(ny / Hb) bands of near-equal height.
+  std::vector<Sub> subtiles(int Hb) const {
+    std::vector<Sub> s;
     for (size_t k = 0; k < doms.size(); ++k) {
-      const int tx = (doms[k].nx + tile2_x - 1) / tile2_x;
-      const int ty = (doms[k].ny + tile2_rows - 1) / tile2_rows;
-      for (int y = 0; y < ty; ++y)
-        for (int x = 0; x < tx; ++x) t.push_back(BatchTile{static_cast<int>(k), x, y, 0});
+      const int nx = doms[k].nx, ny = doms[k].ny;
+      const int nty = (ny + Hb - 1) / Hb, H = (ny + nty - 1) / nty;
+      for (int y = 0; y < ny; y += H)
+        for (int x = 0; x < nx; x += tile2_x)
+          s.push_back(Sub{static_cast<int>(k), x, y, std::min(H, ny - y),
+                          std::min(tile2_x, (nx - x + 63) / 64 * 64)});
     }
-    ntiles2 = static_cast<int>(t.size());
-    if (t.size() > tiles2_cap) {
-      if (dtiles2) LBM_HIP_TRY(hipFree(dtiles2));
-      dtiles2 = nullptr;
-      if (hipMalloc(&dtiles2, t.size() * sizeof(BatchTile)) != hipSuccess)
-        return fail(LBM_ENOMEM, "device allocation of the two-step tile list failed");
-      tiles2_cap = t.size();
+    return s;
+  }
+
+  // First-fit packing, tallest sub-tiles first, into items of block2 lanes: a sub-tile joins the
+  // first open item with room whose tallest sub-tile is at most kSlack rows taller (the item
+  // lasts as long as its tallest walk), else opens one. The items come out tallest first, the
+  // order the workgroups are dispatched in: the short ones fill the last round.
+  static constexpr int kSlack = 4;
+  std::vector<Item> pack(std::vector<Sub>& s) const {
+    std::stable_sort(s.begin(), s.end(), [](const Sub& a, const Sub& b) {
+      return a.rows != b.rows ? a.rows > b.rows : a.width > b.width;
+    });
+    std::vector<Item> items;
+    std::vector<size_t> open;
+    for (size_t i = 0; i < s.size(); ++i) {
+      const int nr = s[i].rows + 2;
+      open.erase(std::remove_if(open.begin(), open.end(),
+                                [&](size_t j) { return items[j].nr_max - nr > kSlack || items[j].free < 64; }),
+                 open.end());
+      bool placed = false;
+      for (size_t j : open)
+        if (items[j].free >= s[i].width) {
+          items[j].free -= s[i].width;
+          items[j].subs.push_back(static_cast<int>(i));
+          placed = true;
+          break;
+        }
+      if (!placed) {
+        items.push_back(Item{nr, block2 - s[i].width, {static_cast<int>(i)}});
+        open.push_back(items.size() - 1);
+      }
     }
-    LBM_HIP_TRY(hipMemcpy(dtiles2, t.data(), t.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
+    return items;
+  }
+
+  // Modelled time of the items on `slots` resident workgroups, dispatched in order each to the
+  // first free slot: an item lasts its tallest walk's step-1 rows.
+  static double makespan(const std::vector<Item>& items, long slots) {
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+    for (long k = 0; k < slots; ++k) free_at.push(0.0);
+    double end = 0.0;
+    for (const Item& it : items) {
+      const double t0 = free_at.top();
+      free_at.pop();
+      free_at.push(t0 + it.nr_max);
+      end = std::max(end, t0 + it.nr_max);
+    }
+    return end;
+  }
+
+  // Band height when tile2_rows is 0: the candidate 16 .. 96 whose packed items the schedule
+  // model finishes soonest (ties: the taller, fewer rows re-read), on the resident workgroups
+  // of the two-step kernel.
+  int model_rows() const {
+    int per_cu = prec == LBM_DOUBLE ? batch_step2_resident<double>(block2, store_nt != 0, one_rsrc)
+                                    : batch_step2_resident<float>(block2, store_nt != 0, one_rsrc);
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long slots = static_cast<long>(std::max(per_cu, 1)) * std::max(cus, 1);
+    int best = 32;
+    double best_t = 0.0;
+    for (int Hb = 96; Hb >= 16; Hb -= 4) {
+      std::vector<Sub> s = subtiles(Hb);
+      const double tm = makespan(pack(s), slots);
+      if (best_t == 0.0 || tm < best_t * 0.999) {
+        best_t = tm;
+        best = Hb;
+      }
+    }
+    return best;
+  }
+
+  // The two-step list: one entry per wave of every item (BatchWave).
+  int build_tiles2() {
+    rows2 = tile2_rows > 0 ? tile2_rows : model_rows();
+    std::vector<Sub> s = subtiles(rows2);
+    const std::vector<Item> items = pack(s);
+    const int nw = block2 / 64;
+    const size_t lds_cap = batch_step2_lds_bytes(block2, rs()) / rs();
+    std::vector<BatchWave> w(items.size() * nw);
+    double cell_rows = 0.0, lane_rows = 0.0;
+    for (size_t i = 0; i < items.size(); ++i) {
+      BatchWave* e = w.data() + i * nw;
+      int lane = 0;
+      size_t lds = 0;
+      for (int j : items[i].subs) {
+        const Sub& u = s[j];
+        for (int v = lane / 64; v < (lane + u.width) / 64; ++v)
+          e[v] = BatchWave{u.domain, u.x0, u.y0, u.rows, lane, u.width, static_cast<int>(lds), items[i].nr_max};
+        lane += u.width;
+        lds += step2_region_elems(u.width);
+        cell_rows += static_cast<double>(std::min(u.width, doms[u.domain].nx - u.x0)) * (u.rows + 2);
+      }
+      if (lane > block2 || lds > lds_cap) return fail(LBM_EINVAL, "two-step item packing overflow");
+      for (int v = lane / 64; v < nw; ++v) e[v] = BatchWave{-1, 0, 0, 0, 0, 0, 0, items[i].nr_max};
+      lane_rows += static_cast<double>(block2) * items[i].nr_max;
+    }
+    fill2 = lane_rows > 0.0 ? cell_rows / lane_rows : 0.0;
+    nitems2 = static_cast<int>(items.size());
+    if (w.size() > waves_cap) {
+      if (dwaves) LBM_HIP_TRY(hipFree(dwaves));
+      dwaves = nullptr;
+      if (hipMalloc(&dwaves, w.size() * sizeof(BatchWave)) != hipSuccess)
+        return fail(LBM_ENOMEM, "device allocation of the two-step work list failed");
+      waves_cap = w.size();
+    }
+    LBM_HIP_TRY(hipMemcpy(dwaves, w.data(), w.size() * sizeof(BatchWave), hipMemcpyHostToDevice));
     drop_graphs();
     return LBM_OK;
   }
 
-  bool two_step() const {
-    return steps_per_launch == 2 && step2_lds_bytes(tile2_x, rs()) <= 64 * 1024;
-  }
+  bool two_step() const { return steps_per_launch == 2; }
 
   template <class T>
   BatchParams<T> params2(bool flip) const {
     BatchParams<T> p = params<T>(flip);
-    p.tiles = dtiles2;
-    p.ntiles = ntiles2;
-    p.tile_rows = tile2_rows;
+    p.waves = dwaves;
+    p.ntiles = nitems2;
     return p;
   }
 
   int launch2(bool flip) {
     const hipError_t e = prec == LBM_DOUBLE
-                             ? launch_batch_step2(params2<double>(flip), tile2_x, workgroups2, stream)
-                             : launch_batch_step2(params2<float>(flip), tile2_x, workgroups2, stream);
+                             ? launch_batch_step2(params2<double>(flip), block2, workgroups2, stream)
+                             : launch_batch_step2(params2<float>(flip), block2, workgroups2, stream);
     if (e != hipSuccess) return fail(LBM_EHIP, std::string("two-step batch launch: ") + hipGetErrorString(e));
     return LBM_OK;
   }
@@ -338,7 +451,8 @@ int lbm_batch_create(int n, const lbm_domain_desc* domains, int precision, lbm_b
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   b->workgroups = std::max(1, cus) * 8;
-  b->tile2_x = precision == LBM_DOUBLE ? 128 : 256;
+  // the two-step kernel's LDS ring holds block2 + 2 columns of six planes per ring row
+  b->block2 = b->tile2_x = precision == LBM_DOUBLE ? 256 : 512;
   const size_t bytes = b->total_elems * b->rs();
   // segment flags in whole dwords: the two-step kernel reads a flag with a scalar dword load
   if (hipMalloc(&b->g, bytes) != hipSuccess || hipMalloc(&b->gn, bytes) != hipSuccess ||
@@ -433,16 +547,15 @@ int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_
   if (!b) return fail(LBM_EINVAL, "null batch handle");
   if (steps_per_launch != 1 && steps_per_launch != 2)
     return fail(LBM_EINVAL, "steps_per_launch must be 1 or 2");
-  if (tile_x != 0 && (tile_x < 64 || tile_x > 512 || tile_x % 64))
-    return fail(LBM_EINVAL, "tile_x must be 0 or a multiple of 64 in 64..512");
-  if (tile_x != 0 && step2_lds_bytes(tile_x, b->rs()) > 64 * 1024)
-    return fail(LBM_EINVAL, "two-step tile too wide for this precision (LDS)");
-  if (tile_rows < 0) return fail(LBM_EINVAL, "tile_rows must be >= 0");
+  if (tile_x != 0 && (tile_x < 64 || tile_x > b->block2 || tile_x % 64))
+    return fail(LBM_EINVAL, "tile_x must be 0 or a multiple of 64 up to " + std::to_string(b->block2) +
+                                " for this precision");
+  if (tile_rows < -1) return fail(LBM_EINVAL, "tile_rows must be -1 (modelled), 0 (keep) or positive");
   if (workgroups < -1) return fail(LBM_EINVAL, "workgroups must be -1 (keep), 0 or positive");
   LBM_HIP_TRY(hipStreamSynchronize(b->stream));
   b->steps_per_launch = steps_per_launch;
   if (tile_x) b->tile2_x = tile_x;
-  if (tile_rows) b->tile2_rows = tile_rows;
+  if (tile_rows) b->tile2_rows = tile_rows < 0 ? 0 : tile_rows;
   if (workgroups >= 0) b->workgroups2 = workgroups;
   b->drop_graphs();
   return b->build_tiles2();
@@ -453,8 +566,19 @@ int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, in
   if (!b) return fail(LBM_EINVAL, "null batch handle");
   if (steps_per_launch) *steps_per_launch = b->two_step() ? 2 : 1;
   if (tile_x) *tile_x = b->tile2_x;
-  if (tile_rows) *tile_rows = b->tile2_rows;
+  if (tile_rows) *tile_rows = b->rows2;
   if (workgroups) *workgroups = b->workgroups2;
+  return LBM_OK;
+}
+
+int lbm_batch_two_step_layout(const lbm_batch* b, int* threads, int* items, int* band_rows,
+                              int* modelled, double* fill) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  if (threads) *threads = b->block2;
+  if (items) *items = b->nitems2;
+  if (band_rows) *band_rows = b->rows2;
+  if (modelled) *modelled = b->tile2_rows == 0 ? 1 : 0;
+  if (fill) *fill = b->fill2;
   return LBM_OK;
 }
 
@@ -579,14 +703,13 @@ int batch_autotune_trials(lbm_batch* b, int steps, int* tile_x, int* tile_rows, 
     double best2 = -1.0;
     int b2x = b->tile2_x, b2r = b->tile2_rows, b2w = b->workgroups2, b2nt = b->store_nt;
     const int keep_nt = b->store_nt;
-    // One workgroup per tile (persistent grids of 2-4 per CU measured 8-25 % slower on the
-    // seeded domains; lbm_batch_set_steps_per_launch still selects them); at least 8 launches
-    // per candidate, the differences between the best shapes being a few percent.
+    // Sub-tile widths and band heights (0: the modelled height), one workgroup per item; at
+    // least 8 launches per candidate, the differences between the best shapes being a few
+    // percent.
     const int pairs = std::max(8, (steps + 1) / 2);
     for (int nt : {0, 1})
-    for (int tx : {64, 128, 256, 512})
-      for (int r : {16, 32, 64}) {
-        if (step2_lds_bytes(tx, b->rs()) > 64 * 1024) continue;
+    for (int tx : {b->block2 / 2, b->block2})
+      for (int r : {0, 32, 48, 64}) {
         b->tile2_x = tx;
         b->tile2_rows = r;
         b->workgroups2 = 0;

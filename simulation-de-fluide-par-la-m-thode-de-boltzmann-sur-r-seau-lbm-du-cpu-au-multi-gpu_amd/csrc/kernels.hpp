@@ -200,8 +200,13 @@ template <class T>
 hipError_t launch_step2_halo(const StepParams<T>& p, const HaloOut<T>& h, int block_x,
                              hipStream_t stream);
 // LDS bytes per workgroup of launch_step2 (independent of the tile height); elem = bytes of the
-// computed type (4 for float and FP16 storage, 8 for double).
+// values the walk carries (4 float, 8 double, 2 FP16 storage).
 size_t step2_lds_bytes(int block_x, size_t elem);
+// Values of one two-step walk's LDS region for a tile `width` threads wide (its ring and halo
+// entries), and the LDS of a packed batch workgroup of block_x threads (every split into
+// sub-tiles of whole waves fits).
+size_t step2_region_elems(int width);
+size_t batch_step2_lds_bytes(int block_x, size_t elem);
 // Tile height for launch_step2: the tallest of 64, 48, 32, 24, 16 rows whose tile count fills
 // the resident workgroup slots (occupancy of the instance p selects, on the current device)
 // evenly with at least min_waves full waves (16 rows always qualifies), else the best-filling

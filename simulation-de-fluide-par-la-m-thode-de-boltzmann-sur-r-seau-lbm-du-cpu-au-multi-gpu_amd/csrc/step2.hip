@@ -201,23 +201,72 @@ __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, con
 constexpr int kHaloChunk = 16;
 constexpr int kHaloRows = kHaloChunk + 2;
 
-// The walk of one tile: columns x0 .. x0 + blockDim.x - 1 (those inside the grid), output rows
-// y0 .. y0 + H - 1 (those below row_end). Ends with every LDS read done before the next walk's
-// first LDS write crosses a barrier (its halo chunk ends with one).
+// The order of a walk over nr step-1 rows: the first halo chunk (kHaloRows rows), start()
+// (no barrier), rows 0 and 1, then four rows per trip with constant ring slots (2, 3, 0, 1) and a remainder, a halo chunk of
+// kHaloChunk rows before the row that has used up the previous one (trip starts are 2 mod 4,
+// and so are the chunk starts 18, 34, ...). In a walk every halo_chunk(k0, cnt) and
+// row(k, hk, slot, two) ends or contains exactly one barrier, so step2_barriers(nr) counts a
+// walk's barriers from this same schedule.
+template <class Halo, class Start, class Row>
+__device__ __forceinline__ void step2_schedule(const int nr, Halo&& halo_chunk, Start&& start,
+                                               Row&& row) {
+  halo_chunk(0, min(kHaloRows, nr));
+  start();
+  row(0, 0, 0, false);
+  row(1, 1, 1, false);
+  // Four rows per trip: constant ring slots (LDS offsets), and the copies between the rows'
+  // register sets disappear (unrolling by 1 or 2, or capping the registers at 80 for a fourth
+  // fp32 workgroup per CU, measured 3-5 % slower). A chunk's entries are dead once its last
+  // row has passed its barrier.
+  int k = 2, hk = 2, hcap = kHaloRows;
+  for (; k + 4 <= nr; k += 4, hk += 4) {
+    if (hk == hcap) {
+      halo_chunk(k, min(kHaloChunk, nr - k));
+      hk = 0;
+      hcap = kHaloChunk;
+    }
+    row(k, hk, 2, true);
+    row(k + 1, hk + 1, 3, true);
+    row(k + 2, hk + 2, 0, true);
+    row(k + 3, hk + 3, 1, true);
+  }
+  for (; k < nr; ++k, ++hk) {
+    if (hk == hcap) {
+      halo_chunk(k, min(kHaloChunk, nr - k));
+      hk = 0;
+      hcap = kHaloChunk;
+    }
+    row(k, hk, k & (kRingSlots - 1), true);
+  }
+}
+
+// Barriers of a walk over nr step-1 rows (wave-uniform).
+__device__ __forceinline__ int step2_barriers(const int nr) {
+  int n = 0;
+  step2_schedule(nr, [&](int, int) { ++n; }, [] {}, [&](int, int, int, bool) { ++n; });
+  return n;
+}
+
+// The walk of one tile by W threads (thread t = 0 .. W - 1, W a multiple of 64, whole waves):
+// columns x0 .. x0 + W - 1 (those inside the grid), output rows y0 .. y0 + H - 1 (those below
+// row_end), with the LDS region at smem (step2_region_elems(W) values). Ends with every LDS
+// read done before the next walk's first LDS write crosses a barrier (its halo chunk ends with
+// one). Its barriers: step2_barriers(min(y0 + H, row_end) - y0 + 2).
 // A whole periodic domain has row_end = ny; a strip's owned rows [row0, row_end) with two ghost
 // rows on each side (row0 >= 2, row_end <= ny - 2) read rows row0 - 2 .. row_end + 1 only, so
 // no row wraps and the ghost rows are plain inputs.
-template <class P, class PS, int kStoreAux, bool kHalo = false>
+template <class P, class PS, int kStoreAux, bool kHalo = false, bool kWhole = false,
+          bool kAhead = false>
 __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, const P& cells,
                                            const int x0, const int y0, const int H,
-                                           const int row_end, unsigned char* smem,
+                                           const int row_end, unsigned char* smem, const int W,
+                                           const int t,
                                            const HaloOut<typename P::S>* hout = nullptr) {
   using T = typename P::Ring;  // the values the walk carries, in registers and in LDS
   using S = typename P::S;
-  const int W = static_cast<int>(blockDim.x), WP = W + 2;
+  const int WP = W + 2;
   T* ring = reinterpret_cast<T*>(smem);                  // [kRingSlots][kRingPlanes][W + 2]
   T* halo = ring + kRingSlots * kRingPlanes * WP;        // [2][kHaloRows][3]
-  const int t = static_cast<int>(threadIdx.x);
   const int nx = p.nx, ny = p.ny;
   const int wv = min(W, nx - x0);  // valid columns of this tile
   const int y1 = min(y0 + H, row_end);
@@ -231,8 +280,14 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   // halo[side][k - k0]; then a barrier. A loop with a wave-uniform trip count: values the
   // compiler shares with the main loop stay scalar.
   const int cl = x0 == 0 ? nx - 1 : x0 - 1, cr = x0 + wv == nx ? 0 : x0 + wv;
-  auto halo_chunk = [&](const int k0, const int cnt) {
-    for (int j0 = 0; j0 < 2 * cnt; j0 += W) {
+  // kWhole (x0 = 0 and W >= nx): the tile is as wide as its grid and is its own halo. The left
+  // neighbour of column 0 is column nx - 1 and the right one of column nx - 1 is column 0, both
+  // computed by the walk itself (lanes wv - 1 and 0 copy them into the ring's halo columns), so
+  // its halo chunks only keep their barrier. (The values of a solid cell differ, the walk
+  // colliding it and halo_step not, but no step reads them: its fluid neighbours bounce back.)
+  constexpr bool whole = kWhole;
+  auto halo_chunk = [&](const int k0, const int cnt) __attribute__((always_inline)) {
+    for (int j0 = 0; j0 < (whole ? 0 : 2 * cnt); j0 += W) {
       const int j = j0 + t;
       if (j < 2 * cnt) {
         const int side = j >= cnt ? 1 : 0, kk = j - side * cnt;
@@ -240,12 +295,17 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
         halo_step(p, cells, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v);
         T* h = halo + (side * kHaloRows + kk) * 3;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) h[q] = v[side ? kRightPlanes[q] : kLeftPlanes[q]];
+        for (int q = 0; q < 3; ++q) {
+          // both candidates as register values (an empty asm): a select of two elements of v
+          // would otherwise become v[index] with the index selected, a compare chain per plane
+          T a = v[kLeftPlanes[q]], b = v[kRightPlanes[q]];
+          asm volatile("" : "+v"(a), "+v"(b));
+          h[q] = side ? b : a;
+        }
       }
     }
     __syncthreads();
   };
-  halo_chunk(0, min(kHaloRows, nr));
 
   // Idle lanes (past the tile's last column) shadow the last column in everything, their ring
   // column included: they compute its values and store them to its address, the same bits.
@@ -269,7 +329,30 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
 #pragma unroll
     for (int i = 0; i < kQ; ++i) out[i] = g.template load_v<S>(i, cols.of(kCx[i]), rw.of(kCy[i]));
   };
-  pull(uniform_rows_of(wrap_y(y0 - 1), nx, ny, B), cur);
+  // kAhead: the bounce-back data of a step-1 row (its segment flag, link codes and own values)
+  // is loaded with the pulls, one row before its use, so that a row next to a wall or an
+  // obstacle does not wait for its loads; otherwise it is loaded and waited for in its row.
+  uint32_t bflag = 0, blink = 0;
+  T bown[kQ];
+  if constexpr (kAhead) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) bown[i] = T(0);
+  }
+  auto bounce_data = [&](const int r, const Rows& rw) __attribute__((always_inline)) {
+    bflag = uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6));
+    if (bflag) {
+      blink = p.links[static_cast<size_t>(r) * nx + c];
+#pragma unroll
+      for (int i = 1; i < kQ; ++i) bown[i] = g.template load_v<S>(kOpp[i], cols.c, rw.c);
+    }
+  };
+  // after the first halo chunk: the pulls of step-1 row 0
+  auto start = [&] {
+    const int r0 = wrap_y(y0 - 1);
+    const Rows rows0 = uniform_rows_of(r0, nx, ny, B);
+    pull(rows0, cur);
+    if constexpr (kAhead) bounce_data(r0, rows0);
+  };
 
   // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
   // are issued first and nothing between them and their use branches around a vector memory
@@ -281,14 +364,24 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     const int r = wrap_y(y0 - 1 + k);
     const Rows rows = uniform_rows_of(r, nx, ny, B);
     T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
-    pull(uniform_rows_of(k + 1 < nr ? wrap_y(y0 + k) : r, nx, ny, B), nxt);
+    const int rn = k + 1 < nr ? wrap_y(y0 + k) : r;
+    const Rows rows_n = uniform_rows_of(rn, nx, ny, B);
+    pull(rows_n, nxt);
     // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
     // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) v[i] = cur[i];
     uint32_t link = 0;
-    if (uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6))) {
+    if constexpr (kAhead) {
+      link = bflag ? blink : 0u;
+      if (bflag) {
+#pragma unroll
+        for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? bown[i] : v[i];
+      }
+      // the next row's (its loads complete by the next row's wait for its pulls)
+      if (k + 1 < nr) bounce_data(rn, rows_n);
+    } else if (uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6))) {
       link = p.links[static_cast<size_t>(r) * nx + c];
       T own[kQ];
 #pragma unroll
@@ -301,14 +394,24 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
 #pragma unroll
     for (int i : {1, 3, 5, 6, 7, 8}) sl[ring_index(i) * WP + tt + 1] = v[i];
     if (t == 0) {
-      const T* h = halo + hk * 3;
+      if (whole) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) sl[ring_index(kLeftPlanes[q]) * WP] = h[q];
+        for (int q = 0; q < 3; ++q) sl[ring_index(kRightPlanes[q]) * WP + wv + 1] = v[kRightPlanes[q]];
+      } else {
+        const T* h = halo + hk * 3;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sl[ring_index(kLeftPlanes[q]) * WP] = h[q];
+      }
     }
     if (t == wv - 1) {
-      const T* h = halo + (kHaloRows + hk) * 3;
+      if (whole) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) sl[ring_index(kRightPlanes[q]) * WP + wv + 1] = h[q];
+        for (int q = 0; q < 3; ++q) sl[ring_index(kLeftPlanes[q]) * WP] = v[kLeftPlanes[q]];
+      } else {
+        const T* h = halo + (kHaloRows + hk) * 3;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sl[ring_index(kRightPlanes[q]) * WP + wv + 1] = h[q];
+      }
     }
     __syncthreads();
     if (two) {
@@ -391,33 +494,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     prev_link = link;
   };
 
-  row(0, 0, 0, false);
-  row(1, 1, 1, false);
-  // Four rows per trip: constant ring slots (LDS offsets), and the copies between the rows'
-  // register sets disappear (unrolling by 1 or 2, or capping the registers at 80 for a fourth
-  // fp32 workgroup per CU, measured 3-5 % slower). Halo chunks: rows 0 .. kHaloRows - 1, then
-  // kHaloChunk rows from kHaloRows + j * kHaloChunk (trip starts: 2 mod 4); a chunk's entries
-  // are dead once its last row has passed its barrier.
-  int k = 2, hk = 2, hcap = kHaloRows;
-  for (; k + 4 <= nr; k += 4, hk += 4) {  // ring slots 2, 3, 0, 1
-    if (hk == hcap) {
-      halo_chunk(k, min(kHaloChunk, nr - k));
-      hk = 0;
-      hcap = kHaloChunk;
-    }
-    row(k, hk, 2, true);
-    row(k + 1, hk + 1, 3, true);
-    row(k + 2, hk + 2, 0, true);
-    row(k + 3, hk + 3, 1, true);
-  }
-  for (; k < nr; ++k, ++hk) {
-    if (hk == hcap) {
-      halo_chunk(k, min(kHaloChunk, nr - k));
-      hk = 0;
-      hcap = kHaloChunk;
-    }
-    row(k, hk, k & (kRingSlots - 1), true);
-  }
+  step2_schedule(nr, halo_chunk, start, row);
 }
 
 template <class T, class PS, int kStoreAux, bool kHalo = false>
@@ -430,7 +507,8 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p, const
   const PlainCells<T> cells{p.omega, p.fp, p.fx, p.fy};
   step2_walk<PlainCells<T>, PS, kStoreAux, kHalo>(d, cells, tx * static_cast<int>(blockDim.x),
                                                   p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
-                                                  p.row0 + p.nrows, smem, &h);
+                                                  p.row0 + p.nrows, smem, static_cast<int>(blockDim.x),
+                                                  static_cast<int>(threadIdx.x), &h);
 }
 
 // FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
@@ -444,25 +522,44 @@ __global__ void __launch_bounds__(512) step2_fp16_kernel(const StepParamsH p, co
   const Step2View<uint16_t> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
   const HalfCells cells{p.col, p.seed, seed2};
   step2_walk<HalfCells, PS, kAuxNone>(d, cells, tx * static_cast<int>(blockDim.x),
-                                      ty * p.rows_per_tile, p.rows_per_tile, p.ny, smem);
+                                      ty * p.rows_per_tile, p.rows_per_tile, p.ny, smem,
+                                      static_cast<int>(blockDim.x), static_cast<int>(threadIdx.x));
 }
 
-// Batched domains: a grid of G workgroups walks the two-step work-list (domain, column tile, row
-// tile of p.tile_rows rows, in domain order), workgroup b taking tiles b, b + G, b + 2G, ... with
-// each tile's domain descriptor; G = the number of tiles is one workgroup per tile (the fastest
-// on the 256 seeded domains: 90.8k against 75-85k MLUPS with persistent grids of 2-4 per CU). One
-// walk's last LDS reads are behind the next walk's first halo-chunk barrier.
+// Batched domains: a grid of G workgroups walks the packed two-step work items, workgroup b
+// taking items b, b + G, b + 2G, ... Each wave reads its own entry of the item (scalar loads):
+// the waves of one sub-tile walk it together as a workgroup of `width` threads would, with
+// their own LDS region; then every wave pads its barriers to those of the item's tallest
+// sub-tile (a wave without a sub-tile only pads), so that all waves of the workgroup meet the
+// same number of barriers, and one more barrier ends the item before the next one's sub-tiles
+// reuse the LDS.
 template <class T, class PS, int kStoreAux>
 __global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p) {
   extern __shared__ __align__(16) unsigned char smem[];
-  for (int t = static_cast<int>(blockIdx.x); t < p.ntiles; t += static_cast<int>(gridDim.x)) {
-    const BatchTile tile = load_uniform(p.tiles + t);
-    const BatchDomain<T> dm = load_uniform(p.domains + tile.domain);
-    const Step2View<T> d{p.g + dm.offset, p.gn + dm.offset, p.links + dm.cell_offset,
-                         p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg};
-    const PlainCells<T> cells{dm.omega, dm.fp, dm.fx, dm.fy};
-    step2_walk<PlainCells<T>, PS, kStoreAux>(d, cells, tile.tx * static_cast<int>(blockDim.x),
-                                             tile.ty * p.tile_rows, p.tile_rows, dm.ny, smem);
+  const int nw = static_cast<int>(blockDim.x) >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+  for (int it = static_cast<int>(blockIdx.x); it < p.ntiles; it += static_cast<int>(gridDim.x)) {
+    const BatchWave w = load_uniform(p.waves + static_cast<size_t>(it) * nw + wave);
+    const int total = step2_barriers(w.nr_max);
+    int done = 0;
+    if (w.domain >= 0) {
+      const BatchDomain<T> dm = load_uniform(p.domains + w.domain);
+      const Step2View<T> d{p.g + dm.offset, p.gn + dm.offset, p.links + dm.cell_offset,
+                           p.segflags + dm.seg_offset, dm.N, dm.nx, dm.ny, dm.nseg};
+      const PlainCells<T> cells{dm.omega, dm.fp, dm.fx, dm.fy};
+      unsigned char* region = smem + static_cast<size_t>(w.lds) * sizeof(T);
+      const int lt = static_cast<int>(threadIdx.x) - w.lane0;
+      // bounce-back data one row ahead: walls and obstacles are common in small domains
+      if (w.x0 == 0 && w.width >= dm.nx)  // a whole domain's width: its own halo
+        step2_walk<PlainCells<T>, PS, kStoreAux, false, true, true>(d, cells, 0, w.y0, w.rows,
+                                                                    dm.ny, region, w.width, lt);
+      else
+        step2_walk<PlainCells<T>, PS, kStoreAux, false, false, true>(d, cells, w.x0, w.y0, w.rows,
+                                                                     dm.ny, region, w.width, lt);
+      done = step2_barriers(w.rows + 2);
+    }
+    for (; done < total; ++done) __syncthreads();
+    __syncthreads();
   }
 }
 
@@ -498,8 +595,17 @@ unsigned step2_grid(const StepParams<T>& p, int block_x) {
 
 }  // namespace
 
-size_t step2_lds_bytes(int block_x, size_t elem) {
-  return (static_cast<size_t>(kRingSlots) * kRingPlanes * (block_x + 2) + 2 * kHaloRows * 3) * elem;
+size_t step2_region_elems(int width) {
+  return static_cast<size_t>(kRingSlots) * kRingPlanes * (width + 2) + 2 * kHaloRows * 3;
+}
+
+size_t step2_lds_bytes(int block_x, size_t elem) { return step2_region_elems(block_x) * elem; }
+
+size_t batch_step2_lds_bytes(int block_x, size_t elem) {
+  // the most sub-tiles a workgroup can hold: one per wave
+  return (static_cast<size_t>(kRingSlots) * kRingPlanes * block_x +
+          static_cast<size_t>(block_x / 64) * (2 * kRingSlots * kRingPlanes + 2 * kHaloRows * 3)) *
+         elem;
 }
 
 template <class T>
@@ -547,9 +653,9 @@ hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, int workgrou
                               hipStream_t stream) {
   // every domain periodic with a uniform force and the plane tail (the batch layout has both)
   if (p.ntiles <= 0) return hipSuccess;
-  if (p.tile_rows < 1 || block_x < 64 || block_x > 512 || block_x % 64 || workgroups < 0)
+  if (!p.waves || block_x < 64 || block_x > 512 || block_x % 64 || workgroups < 0)
     return hipErrorInvalidValue;
-  const size_t lds = step2_lds_bytes(block_x, sizeof(T));
+  const size_t lds = batch_step2_lds_bytes(block_x, sizeof(T));
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const unsigned grid =
       static_cast<unsigned>(workgroups > 0 ? std::min(workgroups, p.ntiles) : p.ntiles);
@@ -568,6 +674,26 @@ hipError_t launch_batch_step2(const BatchParams<T>& p, int block_x, int workgrou
 }
 template hipError_t launch_batch_step2<float>(const BatchParams<float>&, int, int, hipStream_t);
 template hipError_t launch_batch_step2<double>(const BatchParams<double>&, int, int, hipStream_t);
+
+template <class T>
+int batch_step2_resident(int block_x, bool store_nt, bool one_rsrc) {
+  const size_t lds = batch_step2_lds_bytes(block_x, sizeof(T));
+  int n = 0;
+  hipError_t e;
+  if (one_rsrc)
+    e = store_nt ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step2_kernel<T, PlanesOne, kAuxNt>, block_x, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step2_kernel<T, PlanesOne, kAuxNone>, block_x, lds);
+  else
+    e = store_nt ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step2_kernel<T, Planes, kAuxNt>, block_x, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step2_kernel<T, Planes, kAuxNone>, block_x, lds);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+template int batch_step2_resident<float>(int, bool, bool);
+template int batch_step2_resident<double>(int, bool, bool);
 
 // Tile height for a two-step kernel instance: `kernel` with block_x threads and `lds` bytes.
 template <class K>

@@ -75,3 +75,30 @@ def test_counter_values_of_one_dispatch_are_summed_and_skip(tmp_path, counters):
     assert counters.read_counters(tmp_path / "g.csv")[0]["value"] == 4.0
     with pytest.raises(ValueError, match="no dispatch"):
         counters.summarize(r, r, "nothing_like_this", 10, 1, 72)
+
+
+def test_launch_of_several_dispatches(tmp_path, counters):
+    """A launch made of several dispatches (the batch: one per tile-width class): the last whole
+    launches are selected and the per-dispatch mean scaled back to a launch."""
+    cal = 2_415_919_104
+    cells = 3_000
+    # two tuning dispatches to leave out, then 4 launches of 3 dispatches (12 B, 24 B, 36 B
+    # per cell of the launch's read bytes: 72 B over the three, x 2 in FETCH)
+    parts = [12 * cells, 24 * cells, 36 * cells]
+    fetch = [(COPY, "FETCH_SIZE", cal / 1024 / 2)] + [(STEP, "FETCH_SIZE", 1e9)] * 2 + \
+            [(STEP, "FETCH_SIZE", b / 1024 / 2) for _ in range(4) for b in parts]
+    write = [(COPY, "WRITE_SIZE", cal / 1024)] + [(STEP, "WRITE_SIZE", 1e9)] * 2 + \
+            [(STEP, "WRITE_SIZE", b / 1024) for _ in range(4) for b in parts]
+    write_csv(tmp_path / "f.csv", fetch)
+    write_csv(tmp_path / "w.csv", write)
+    s = counters.summarize(counters.read_counters(tmp_path / "f.csv"),
+                           counters.read_counters(tmp_path / "w.csv"), "step2_kernel", cells, 2,
+                           72 * 2, last=12, dispatches_per_launch=3)
+    assert s["dispatches"] == {"fetch": 12, "write": 12}
+    assert s["read_bytes_per_launch"] == pytest.approx(72 * cells, rel=1e-9)
+    assert s["write_bytes_per_launch"] == pytest.approx(72 * cells, rel=1e-9)
+    assert s["traffic_vs_algorithmic"] == pytest.approx(1.0)
+    with pytest.raises(ValueError, match="whole number"):
+        counters.summarize(counters.read_counters(tmp_path / "f.csv"),
+                           counters.read_counters(tmp_path / "w.csv"), "step2_kernel", cells, 2,
+                           144, last=10, dispatches_per_launch=3)

@@ -97,10 +97,13 @@ def test_batch_rejects_invalid_domains(lbm):
 
 @pytest.mark.parametrize("prec", ("float", "double"))
 def test_two_step_batch_matches_one_step_batch(lbm, oracle_mod, prec):
-    """Two steps per launch over the batch (one workgroup per tile, per-domain descriptors):
-    tile shapes from narrow to wider than most domains, graphs of both parities, direct pairs and
-    an odd remainder; every domain equals the one-step batch and, for two, the oracle."""
-    doms = lbm.random_domains(seed=21, n=30, nx_range=(20, 700), ny_range=(9, 200),
+    """Two steps per launch over the batch (per-domain sub-tiles, one launch per pair of steps,
+    several sub-tiles of different domains and heights packed into one workgroup): sub-tile
+    widths from one wave to wider than most domains (domains up to 1100 columns in two or
+    three column tiles), bands of uneven heights, one workgroup per item and persistent grids,
+    graphs of both parities, direct pairs and an odd remainder; then the modelled band height.
+    Every domain equals the one-step batch and, for two, the oracle."""
+    doms = lbm.random_domains(seed=21, n=30, nx_range=(20, 1100), ny_range=(9, 300),
                               radius_range=(2, 12))
     init = (1.0, 0.02, -0.01)
     out = []
@@ -109,17 +112,24 @@ def test_two_step_batch_matches_one_step_batch(lbm, oracle_mod, prec):
         b.set_graph_steps(6)
         if spl == 2:
             # one workgroup per tile, and persistent grids smaller than the tile count
-            for tile, wg, steps in (((64, 16), 0, 6), ((256, 32), 7, 5), ((512, 64), 300, 7)):
+            for tile, wg, steps in (((64, 16), 0, 4), ((192, 48), 7, 5), ((512, 64), 300, 7),
+                                    ((320, 100), 0, 2), ((128, 5), 0, 2), ((448, -1), 0, 2)):
                 if prec == "double":  # its LDS ring holds twice the bytes: at most 256 columns
                     tile = (min(tile[0], 256), tile[1])
                 b.set_steps_per_launch(2, *tile, wg)
+                lay = b.two_step_layout()
+                rows = lay["band_rows"]
+                assert lay["modelled"] == (tile[1] == -1)
+                assert rows == tile[1] if tile[1] > 0 else 16 <= rows <= 96
                 assert b.steps_per_launch() == {"steps_per_launch": 2, "tile_x": tile[0],
-                                                "tile_rows": tile[1], "workgroups": wg}
+                                                "tile_rows": rows, "workgroups": wg}
+                assert lay["threads"] == (256 if prec == "double" else 512)
+                assert 0.3 < lay["fill"] <= 1.0 and lay["items"] >= 1
                 b.run(steps)
         else:
             b.set_steps_per_launch(1)
             assert b.steps_per_launch()["steps_per_launch"] == 1
-            b.run(18)
+            b.run(22)
         out.append([b.populations(k) for k in range(len(doms))])
         b.close()
     for k in range(len(doms)):
@@ -128,5 +138,5 @@ def test_two_step_batch_matches_one_step_batch(lbm, oracle_mod, prec):
         d = doms[k]
         o = oracle_mod.Oracle(d["nx"], d["ny"], d["tau"], d["fx"], d["fy"], d["solid"], prec, "v0")
         o.init_equilibrium(*init)
-        o.run(18)
+        o.run(22)
         assert np.count_nonzero(bits(o.populations()) != bits(out[0][k])) == 0, k
