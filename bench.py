@@ -44,24 +44,26 @@ import __graft_entry__  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {"float": 72, "double": 144, "fp16": 36}
 TRAFFIC_SUMMARY = ROOT / "profiles" / "step2_fp32_8192x8192.traffic.md"
+BATCH_TRAFFIC_SUMMARY = ROOT / "profiles" / "batch_step2_256_domains.traffic.md"
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_summary(lbm):
-    """The two-step fp32 kernel's calibrated HBM traffic and VALU count per launch, from the
-    summary the package's counter tool (counters.py: rocprofv3 FETCH_SIZE / WRITE_SIZE /
-    SQ_INSTS_VALU passes) wrote; `stale` when the device sources changed since."""
+def traffic_summary(lbm, path=None):
+    """A kernel's calibrated HBM traffic and VALU count per launch (default: the two-step fp32
+    kernel), from the summary the package's counter tool (counters.py: rocprofv3 FETCH_SIZE /
+    WRITE_SIZE / SQ_INSTS_VALU passes) wrote; `stale` when the device sources changed since."""
     import importlib
 
+    path = TRAFFIC_SUMMARY if path is None else path
     counters = importlib.import_module(lbm.__name__ + ".counters")
-    s = counters.read_summary(TRAFFIC_SUMMARY)
+    s = counters.read_summary(path)
     if not s:
         return None
     s["stale"] = s.get("source_tag") != counters.source_tag()
-    s["file"] = str(TRAFFIC_SUMMARY.relative_to(ROOT))
+    s["file"] = str(path.relative_to(ROOT))
     return s
 
 
@@ -291,7 +293,18 @@ def form_batched(lbm, args):
             else "persistent",
             "one_step": {"order": order, "streaming_stores": nt, "tile_x": tx, "tile_rows": tr,
                          "workgroups": wg},
-            "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / per_launch / 1e9, 1)}
+            "hbm_gbs_algorithmic": round(mlups * 1e6 * 72 / per_launch / 1e9, 1),
+            "traffic": batch_traffic(lbm)}
+
+
+def batch_traffic(lbm):
+    """Calibrated traffic of the batched two-step kernel on these 256 domains (counter tool)."""
+    s = traffic_summary(lbm, BATCH_TRAFFIC_SUMMARY)
+    if not s:
+        return None
+    return {k: s.get(k) for k in ("file", "stale", "source_tag", "traffic_bytes_per_launch",
+                                  "traffic_vs_algorithmic", "read_vs_algorithmic",
+                                  "valu_instructions_per_update")}
 
 
 def form_strip(lbm, args):

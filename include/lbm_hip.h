@@ -276,6 +276,10 @@ int lbm_batch_set_steps_per_launch(lbm_batch* b, int steps_per_launch, int tile_
 /* tile_rows: the band height in use (modelled or set). */
 int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, int* tile_x,
                                    int* tile_rows, int* workgroups);
+/* Threads per workgroup of the two-step kernel (a multiple of 64; at most 512, double 256, the
+ * defaults): every work item packs sub-tiles into that many lanes, and the sub-tile width is
+ * capped to it. */
+int lbm_batch_set_two_step_threads(lbm_batch* b, int threads);
 /* The two-step work list: threads per workgroup, work items, band rows in use, whether they are
  * the modelled choice, and the fill (cell rows over lane rows of the items, 0..1). Any output
  * may be null. */

@@ -128,6 +128,7 @@ def _declare(L):
         "lbm_batch_set_steps_per_launch": ([vp, i, i, i, i], i),
         "lbm_batch_get_steps_per_launch": ([vp, ip, ip, ip, ip], i),
         "lbm_batch_two_step_layout": ([vp, ip, ip, ip, ip, dp], i),
+        "lbm_batch_set_two_step_threads": ([vp, i], i),
         "lbm_adjoint_create": ([i, i, d, _u8p, i, i, i, i, i, i, i, i, C.POINTER(vp)], i),
         "lbm_adjoint_destroy": ([vp], i),
         "lbm_adjoint_set_force_field": ([vp, vp], i),
@@ -667,6 +668,11 @@ class Batch:
                                                    C.byref(w)))
         return {"steps_per_launch": a.value, "tile_x": b.value, "tile_rows": c.value,
                 "workgroups": w.value}
+
+    def set_two_step_threads(self, threads: int):
+        """Threads per workgroup of the two-step kernel (multiple of 64, at most 512; double
+        256); caps the sub-tile width to it."""
+        check(lib().lbm_batch_set_two_step_threads(self._h, int(threads)))
 
     def two_step_layout(self) -> dict:
         """The two-step work list: threads per workgroup, work items, band rows in use, whether

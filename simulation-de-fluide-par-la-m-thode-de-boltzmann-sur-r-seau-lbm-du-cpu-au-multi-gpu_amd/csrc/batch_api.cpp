@@ -571,6 +571,19 @@ int lbm_batch_get_steps_per_launch(const lbm_batch* b, int* steps_per_launch, in
   return LBM_OK;
 }
 
+int lbm_batch_set_two_step_threads(lbm_batch* b, int threads) {
+  if (!b) return fail(LBM_EINVAL, "null batch handle");
+  const int most = b->prec == LBM_DOUBLE ? 256 : 512;
+  if (threads < 64 || threads > most || threads % 64)
+    return fail(LBM_EINVAL, "threads must be a multiple of 64 up to " + std::to_string(most) +
+                                " for this precision");
+  LBM_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->block2 = threads;
+  b->tile2_x = std::min(b->tile2_x, threads);
+  b->drop_graphs();
+  return b->build_tiles2();
+}
+
 int lbm_batch_two_step_layout(const lbm_batch* b, int* threads, int* items, int* band_rows,
                               int* modelled, double* fill) {
   if (!b) return fail(LBM_EINVAL, "null batch handle");
