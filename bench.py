@@ -407,10 +407,49 @@ def form_cylinder(lbm, args):
     return out
 
 
+def form_edges(lbm, args):
+    """Open x (inlet column, outlet column) and a per-cell force field on the two-step walk
+    against one step per launch: a 4096 x 1024 walled channel with a disc of diameter 64,
+    float and double, 200 timed steps after 20; the field is the adjoint form's (N(0, 1e-7)
+    per cell from seed 0, plus gx = 1e-5)."""
+    import numpy as np
+
+    nx, ny = 4096, 1024
+    solid = lbm.make_channel(nx, ny)
+    lbm.add_disc(solid, nx, ny, 512.0, 512.0, 32.0)
+    rng = np.random.default_rng(0)
+    field = rng.normal(0.0, 1e-7, 2 * nx * ny)
+    field[: nx * ny] += 1e-5
+    out = {"grid": f"{nx}x{ny}", "steps": 200}
+    for case in ("open_x", "force_field"):
+        out[case] = {}
+        for prec in ("float", "double"):
+            rate = {}
+            for spl in (2, 1):
+                s = lbm.Solver(nx, ny, 0.6, 0.0 if case == "force_field" else 1e-6, 0.0, solid, prec)
+                if case == "open_x":
+                    s.set_inflow_outflow(0.05)
+                else:
+                    s.set_force_field(field.astype(np.float32 if prec == "float" else np.float64))
+                s.set_steps_per_launch(spl)
+                assert s.steps_per_launch()["in_use"] == (spl == 2)
+                s.init_equilibrium(1.0, 0.05 if case == "open_x" else 0.0, 0.0)
+                s.run(20)
+                rate[spl] = nx * ny * 200 / s.timed_run(200) / 1e6
+                s.close()
+            out[case][prec] = {"two_step_mlups": round(rate[2], 1),
+                               "single_step_mlups": round(rate[1], 1),
+                               "speedup": round(rate[2] / rate[1], 3)}
+            log(f"{case} {prec}: two steps per launch {rate[2]:.0f} MLUPS, one step "
+                f"{rate[1]:.0f} MLUPS (x{rate[2] / rate[1]:.2f})")
+    return out
+
+
 FORMS = {"single_step": form_single_step, "protocol": form_protocol,
          "copy_ceiling": form_copy_ceiling,
          "precisions": form_precisions,
-         "batched": form_batched, "strip": form_strip, "adjoint": form_adjoint,
+         "batched": form_batched, "strip": form_strip, "edges": form_edges,
+         "adjoint": form_adjoint,
          "cylinder": form_cylinder}
 
 

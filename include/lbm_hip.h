@@ -93,8 +93,9 @@ int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* sing
 
 /* No reference counterpart: temporal blocking. With steps_per_launch = 2 (the default)
  * lbm_run and lbm_timed_run advance pairs of steps in one launch of a two-step kernel wherever
- * it applies (periodic domain, uniform force; float, double or FP16 storage): every population
- * is read from HBM and written once per two steps. The results are bit-identical to single
+ * it applies (float and double: periodic or open x, uniform force or a per-cell field; FP16
+ * storage: periodic x, uniform force): every population is read from HBM and written once per
+ * two steps. The results are bit-identical to single
  * steps (same operations per cell). block_x (64..512, multiple of 64; 0 keeps the current
  * value) and rows set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64,
  * 48, 32, 24, 16 rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps

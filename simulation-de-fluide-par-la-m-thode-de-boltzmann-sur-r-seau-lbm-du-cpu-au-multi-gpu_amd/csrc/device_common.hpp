@@ -187,31 +187,36 @@ __device__ __forceinline__ void pull_streamed(const PS& g, const Rows& rows, con
   for (int i = 0; i < kQ; ++i) v[i] = g.template load<S>(i, cols.of(kCx[i]), rows.of(kCy[i]));
 }
 
+// The open-x inlet / outlet rules of one cell from its link code (solver_v3.cu:66-73,
+// src/common/boundary.hpp:53-70), applied after its bounce-back corrections.
+template <class S, class PS, class C>
+__device__ __forceinline__ void apply_open_edges(uint32_t link, const PS& g, const Rows& rows,
+                                                 const Cols& cols, const C* eq_in, C (&v)[kQ]) {
+  if (link & kLinkInlet) {
+    v[1] = eq_in[1];
+    v[5] = eq_in[5];
+    v[8] = eq_in[8];
+  }
+  if (link & kLinkOutlet) {
+    // What x - 1 = nx - 2 receives in direction i: pulled from (nx - 1, y - cy_i), or
+    // bounced back from opp(i) of cell nx - 2.
+    const unsigned left = cols.c - static_cast<unsigned>(sizeof(S));
+#pragma unroll
+    for (int i : {3, 6, 7})
+      v[i] = (link & outlet_bb_bit(i)) ? g.template load<S>(kOpp[i], left, rows.c)
+                                       : g.template load<S>(i, cols.c, rows.of(kCy[i]));
+  }
+}
+
 // Dirty-segment corrections for one cell, from its link code: bounce-back from the cell's own
-// opposite slot (src/v3_cuda_opt/solver_v3.cu:74-75), and the open-x inlet / outlet rules
-// (solver_v3.cu:66-73, src/common/boundary.hpp:53-70).
+// opposite slot (src/v3_cuda_opt/solver_v3.cu:74-75), then the open-x rules.
 template <class S, bool kOpen, class PS = Planes, class C = typename Stored<S>::C>
 __device__ __forceinline__ void apply_links(uint32_t link, const PS& g, const Rows& rows,
                                             const Cols& cols, const C* eq_in, C (&v)[kQ]) {
 #pragma unroll
   for (int i = 1; i < kQ; ++i)
     if (link & (1u << i)) v[i] = g.template load<S>(kOpp[i], cols.c, rows.c);
-  if constexpr (kOpen) {
-    if (link & kLinkInlet) {
-      v[1] = eq_in[1];
-      v[5] = eq_in[5];
-      v[8] = eq_in[8];
-    }
-    if (link & kLinkOutlet) {
-      // What x - 1 = nx - 2 receives in direction i: pulled from (nx - 1, y - cy_i), or
-      // bounced back from opp(i) of cell nx - 2.
-      const unsigned left = cols.c - static_cast<unsigned>(sizeof(S));
-#pragma unroll
-      for (int i : {3, 6, 7})
-        v[i] = (link & outlet_bb_bit(i)) ? g.template load<S>(kOpp[i], left, rows.c)
-                                         : g.template load<S>(i, cols.c, rows.of(kCy[i]));
-    }
-  }
+  if constexpr (kOpen) apply_open_edges<S>(link, g, rows, cols, eq_in, v);
 }
 
 // Whole-domain gather (periodic y) of the post-streaming populations of fluid cell (x, y);

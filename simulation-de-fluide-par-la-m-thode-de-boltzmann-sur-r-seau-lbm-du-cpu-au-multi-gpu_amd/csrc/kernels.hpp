@@ -188,12 +188,15 @@ struct HaloOut {
 // kPlaneTail): launch_step2 stores the results of solid cells there instead of skipping them.
 constexpr size_t kPlaneTail = 64;
 
-// Two fused steps per launch (step2.hip): periodic domain, uniform force, planes < 4 GiB with
-// the kPlaneTail tail; p.rows_per_tile = output rows per tile, block_x (64..512, a multiple of
-// 64) columns per tile, p.tiles_x = ceil(nx / block_x). Bit-identical to two launches of
-// launch_step (solid cells keep their values in both).
+// Two fused steps per launch (step2.hip): a periodic-y domain (periodic x, or open x with the
+// inlet / outlet rules when open_x, nx >= 3), uniform force or a per-cell field (p.force), or
+// the owned rows of a strip (periodic x, uniform force); planes < 4 GiB with the kPlaneTail
+// tail; p.rows_per_tile = output rows per tile, block_x (64..512, a multiple of 64) columns per
+// tile, p.tiles_x = ceil(nx / block_x). Bit-identical to two launches of launch_step (solid
+// cells keep their values in both).
 template <class T>
-hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream);
+hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream,
+                        bool open_x = false);
 // The same over a strip's owned rows (wrap_y = 0), also storing its halo entries into the
 // receivers (HaloOut). Each receiver's nine planes must span < 4 GiB (one buffer resource).
 template <class T>

@@ -347,7 +347,8 @@ int SolverState::step() {
 }
 
 bool SolverState::two_step_eligible() const {
-  if (steps_per_launch != 2 || open_x || force) return false;
+  // open x and force fields: float and double (FP16 storage keeps one step per launch there)
+  if (steps_per_launch != 2 || ((open_x || force) && prec == LBM_FP16)) return false;
   if (S * storage_size(prec) >= (size_t(1) << 32)) return false;
   // the LDS ring holds computed values: fp32 for FP16 storage
   return step2_lds_bytes(block2_x, prec == LBM_DOUBLE ? 8 : 4) <= 64 * 1024;
@@ -385,13 +386,13 @@ int SolverState::advance(int n) {
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
         p.xcd_bands = order;
-        e = launch_step2(p, block2_x, stream);
+        e = launch_step2(p, block2_x, stream, open_x);
       } else {
         StepParams<double> p = make_step_params<double>(*this);
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
         p.xcd_bands = order;
-        e = launch_step2(p, block2_x, stream);
+        e = launch_step2(p, block2_x, stream, open_x);
       }
       if (e == hipSuccess) {
         std::swap(g, gn);

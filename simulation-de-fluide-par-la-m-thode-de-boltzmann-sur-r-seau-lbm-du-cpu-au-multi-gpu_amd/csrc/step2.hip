@@ -107,6 +107,14 @@ struct Step2View {
 // Cell updates of the walk. Stored type S, computed type C, written word Word; step1() turns a
 // cell's pulled values into its state one step later as the next step will read it, step2()
 // into the words the store writes. `cell` is the cell's index y * nx + x.
+// What an open-x domain or a per-cell force field adds to a walk (float / double cells only):
+// the inlet equilibrium and the force planes (fx at force, fy at force + N).
+template <class T>
+struct WalkEdges {
+  const T* force;
+  T eq_in[kQ];
+};
+
 // float / double: the reference's collision, bit-identical to step_kernel.
 template <class T>
 struct PlainCells {
@@ -120,6 +128,13 @@ struct PlainCells {
   __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
   __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
     collide(v, fx, fy, omega, fp);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) out[i] = v[i];
+  }
+  // with the cell's own force (a per-cell field)
+  __device__ __forceinline__ void step1(Ring (&v)[kQ], T cfx, T cfy) const { collide(v, cfx, cfy, omega, fp); }
+  __device__ __forceinline__ void step2(Ring (&v)[kQ], T cfx, T cfy, Word (&out)[kQ]) const {
+    collide(v, cfx, cfy, omega, fp);
 #pragma unroll
     for (int i = 0; i < kQ; ++i) out[i] = v[i];
   }
@@ -171,9 +186,10 @@ struct HalfCells {
 // wave: the row offsets go into the per-lane offset (a per-lane scalar offset would make the
 // compiler loop over its values). The pull with its bounce-back corrections and the collision,
 // exactly as update_column.
-template <class P, class PS>
+template <class P, class PS, bool kOpen = false, bool kField = false>
 __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, const P& cells,
-                                          const PS& g, int c, int r, typename P::Ring (&v)[kQ]) {
+                                          const PS& g, int c, int r, typename P::Ring (&v)[kQ],
+                                          const WalkEdges<typename P::S>* ex = nullptr) {
   using S = typename P::S;
   constexpr unsigned B = sizeof(S);
   const Rows rows = rows_of(r, p.nx, p.ny, B);
@@ -187,10 +203,31 @@ __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, con
 #pragma unroll
       for (int i = 1; i < kQ; ++i)
         if (link & (1u << i)) v[i] = g.template load_v<S>(kOpp[i], cols.c + rows.c, 0u);
+      if constexpr (kOpen) {  // apply_open_edges with per-lane rows (all in the vector offset)
+        if (link & kLinkInlet) {
+          v[1] = ex->eq_in[1];
+          v[5] = ex->eq_in[5];
+          v[8] = ex->eq_in[8];
+        }
+        if (link & kLinkOutlet) {
+          const unsigned left = cols.c - B;
+#pragma unroll
+          for (int i : {3, 6, 7})
+            v[i] = (link & outlet_bb_bit(i)) ? g.template load_v<S>(kOpp[i], left + rows.c, 0u)
+                                             : g.template load_v<S>(i, cols.c + rows.of(kCy[i]), 0u);
+        }
+      }
     }
   }
-  if (!(link & kLinkSolid))
-    cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(p.nx) + static_cast<uint32_t>(c));
+  if (!(link & kLinkSolid)) {
+    if constexpr (kField) {
+      const S cfx = Stored<S>::load(make_rsrc(ex->force), cols.c + rows.c, 0u);
+      const S cfy = Stored<S>::load(make_rsrc(ex->force + p.N), cols.c + rows.c, 0u);
+      cells.step1(v, cfx, cfy);
+    } else {
+      cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(p.nx) + static_cast<uint32_t>(c));
+    }
+  }
 }
 
 // Halo-column cells are computed for kHaloChunk step-1 rows at a time (the first chunk also
@@ -255,13 +292,19 @@ __device__ __forceinline__ int step2_barriers(const int nr) {
 // A whole periodic domain has row_end = ny; a strip's owned rows [row0, row_end) with two ghost
 // rows on each side (row0 >= 2, row_end <= ny - 2) read rows row0 - 2 .. row_end + 1 only, so
 // no row wraps and the ghost rows are plain inputs.
+// kOpen: an open-x domain (inlet column 0, outlet column nx - 1: the rules of apply_links in
+// both steps, the second one's from the ring); kField: a per-cell force field (each row's force
+// loaded with its pulls, kept for its step 2). Both take ex and float / double cells.
 template <class P, class PS, int kStoreAux, bool kHalo = false, bool kWhole = false,
-          bool kAhead = false>
+          bool kAhead = false, bool kOpen = false, bool kField = false>
 __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, const P& cells,
                                            const int x0, const int y0, const int H,
                                            const int row_end, unsigned char* smem, const int W,
                                            const int t,
-                                           const HaloOut<typename P::S>* hout = nullptr) {
+                                           const HaloOut<typename P::S>* hout = nullptr,
+                                           const WalkEdges<typename P::S>* ex = nullptr) {
+  static_assert(!(kOpen || kField) || (!P::kRowInRegisters && !kAhead && !kWhole),
+                "open edges and force fields: plain cells, bounce-back data in its row");
   using T = typename P::Ring;  // the values the walk carries, in registers and in LDS
   using S = typename P::S;
   const int WP = W + 2;
@@ -292,7 +335,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       if (j < 2 * cnt) {
         const int side = j >= cnt ? 1 : 0, kk = j - side * cnt;
         T v[kQ];
-        halo_step(p, cells, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v);
+        halo_step<P, PS, kOpen, kField>(p, cells, g, side ? cr : cl, wrap_y(y0 - 1 + k0 + kk), v, ex);
         T* h = halo + (side * kHaloRows + kk) * 3;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -346,12 +389,20 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       for (int i = 1; i < kQ; ++i) bown[i] = g.template load_v<S>(kOpp[i], cols.c, rw.c);
     }
   };
+  // kField: the force of step-1 row r (f1), of the row above it (f2, step 2's), of the next row
+  const Rsrc fxr = make_rsrc(kField ? static_cast<const void*>(ex->force) : nullptr);
+  const Rsrc fyr = make_rsrc(kField ? static_cast<const void*>(ex->force + p.N) : nullptr);
+  S f1x = S(0), f1y = S(0), f2x = S(0), f2y = S(0);
   // after the first halo chunk: the pulls of step-1 row 0
   auto start = [&] {
     const int r0 = wrap_y(y0 - 1);
     const Rows rows0 = uniform_rows_of(r0, nx, ny, B);
     pull(rows0, cur);
     if constexpr (kAhead) bounce_data(r0, rows0);
+    if constexpr (kField) {
+      f1x = Stored<S>::load(fxr, cols.c, rows0.c);
+      f1y = Stored<S>::load(fyr, cols.c, rows0.c);
+    }
   };
 
   // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
@@ -367,6 +418,11 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     const int rn = k + 1 < nr ? wrap_y(y0 + k) : r;
     const Rows rows_n = uniform_rows_of(rn, nx, ny, B);
     pull(rows_n, nxt);
+    S fnx = S(0), fny = S(0);
+    if constexpr (kField) {
+      fnx = Stored<S>::load(fxr, cols.c, rows_n.c);
+      fny = Stored<S>::load(fyr, cols.c, rows_n.c);
+    }
     // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
     // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
@@ -388,8 +444,12 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       for (int i = 1; i < kQ; ++i) own[i] = g.template load_v<S>(kOpp[i], cols.c, rows.c);
 #pragma unroll
       for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? own[i] : v[i];
+      if constexpr (kOpen) apply_open_edges<S>(link, g, rows, cols, ex->eq_in, v);
     }
-    cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c));
+    if constexpr (kField)
+      cells.step1(v, f1x, f1y);
+    else
+      cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c));
     T* sl = ring + slot * kRingPlanes * WP;
 #pragma unroll
     for (int i : {1, 3, 5, 6, 7, 8}) sl[ring_index(i) * WP + tt + 1] = v[i];
@@ -444,9 +504,30 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
         for (int i = 1; i < kQ; ++i)
           if (bb & (1u << i)) w[i] = pq[kOpp[i]];
       }
+      if constexpr (kOpen) {
+        // apply_open_edges on the step-1 values: the inlet equilibrium, and at the outlet what
+        // x - 1 receives, R_{t+1}(i, x, q - c_y) (ring rows q, q - 1, q + 1 at column x) or
+        // R_{t+1}(opp(i), x - 1, q) (ring row q at column x - 1)
+        if (prev_link & kLinkInlet) {
+          w[1] = ex->eq_in[1];
+          w[5] = ex->eq_in[5];
+          w[8] = ex->eq_in[8];
+        }
+        if (prev_link & kLinkOutlet) {
+          w[3] = (prev_link & outlet_bb_bit(3)) ? s1[ring_index(kOpp[3]) * WP + tt]
+                                                : s1[ring_index(3) * WP + tt + 1];
+          w[6] = (prev_link & outlet_bb_bit(6)) ? s1[ring_index(kOpp[6]) * WP + tt]
+                                                : s2[ring_index(6) * WP + tt + 1];
+          w[7] = (prev_link & outlet_bb_bit(7)) ? s1[ring_index(kOpp[7]) * WP + tt]
+                                                : s0[ring_index(7) * WP + tt + 1];
+        }
+      }
       typename P::Word out[kQ];
       const int q = y0 - 2 + k;
-      cells.step2(w, static_cast<uint32_t>(q) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c), out);
+      if constexpr (kField)
+        cells.step2(w, f2x, f2y, out);
+      else
+        cells.step2(w, static_cast<uint32_t>(q) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c), out);
       const unsigned so = static_cast<unsigned>(
           __builtin_amdgcn_readfirstlane(static_cast<unsigned>(q * nx) * B));
       const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
@@ -492,12 +573,19 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       cur[i] = nxt[i];
     }
     prev_link = link;
+    if constexpr (kField) {
+      f2x = f1x;
+      f2y = f1y;
+      f1x = fnx;
+      f1y = fny;
+    }
   };
 
   step2_schedule(nr, halo_chunk, start, row);
 }
 
-template <class T, class PS, int kStoreAux, bool kHalo = false>
+template <class T, class PS, int kStoreAux, bool kHalo = false, bool kOpen = false,
+          bool kField = false>
 __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p, const HaloOut<T> h) {
   extern __shared__ __align__(16) unsigned char smem[];
   const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x, p.tiles_x);
@@ -505,10 +593,15 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p, const
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const Step2View<T> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
   const PlainCells<T> cells{p.omega, p.fp, p.fx, p.fy};
-  step2_walk<PlainCells<T>, PS, kStoreAux, kHalo>(d, cells, tx * static_cast<int>(blockDim.x),
-                                                  p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
-                                                  p.row0 + p.nrows, smem, static_cast<int>(blockDim.x),
-                                                  static_cast<int>(threadIdx.x), &h);
+  WalkEdges<T> ex;
+  if constexpr (kOpen || kField) {
+    ex.force = p.force;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) ex.eq_in[i] = p.eq_in[i];
+  }
+  step2_walk<PlainCells<T>, PS, kStoreAux, kHalo, false, false, kOpen, kField>(
+      d, cells, tx * static_cast<int>(blockDim.x), p.row0 + ty * p.rows_per_tile, p.rows_per_tile,
+      p.row0 + p.nrows, smem, static_cast<int>(blockDim.x), static_cast<int>(threadIdx.x), &h, &ex);
 }
 
 // FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
@@ -563,25 +656,39 @@ __global__ void __launch_bounds__(512) batch_step2_kernel(const BatchParams<T> p
   }
 }
 
-template <class T, class PS>
-void launch_planes(const StepParams<T>& p, unsigned grid, int block_x, size_t lds, hipStream_t stream) {
+template <class T, class PS, bool kOpen, bool kField>
+void launch_edges(const StepParams<T>& p, unsigned grid, int block_x, size_t lds, hipStream_t stream) {
   const HaloOut<T> none{};
   if (p.store_nt)
-    step2_kernel<T, PS, kAuxNt><<<grid, block_x, lds, stream>>>(p, none);
+    step2_kernel<T, PS, kAuxNt, false, kOpen, kField><<<grid, block_x, lds, stream>>>(p, none);
   else
-    step2_kernel<T, PS, kAuxNone><<<grid, block_x, lds, stream>>>(p, none);
+    step2_kernel<T, PS, kAuxNone, false, kOpen, kField><<<grid, block_x, lds, stream>>>(p, none);
+}
+
+template <class T, class PS>
+void launch_planes(const StepParams<T>& p, bool open_x, unsigned grid, int block_x, size_t lds,
+                   hipStream_t stream) {
+  const bool field = p.force != nullptr;
+  if (open_x && field)
+    launch_edges<T, PS, true, true>(p, grid, block_x, lds, stream);
+  else if (open_x)
+    launch_edges<T, PS, true, false>(p, grid, block_x, lds, stream);
+  else if (field)
+    launch_edges<T, PS, false, true>(p, grid, block_x, lds, stream);
+  else
+    launch_edges<T, PS, false, false>(p, grid, block_x, lds, stream);
 }
 
 // The checks of launch_step2 (one function for both entry points); returns the grid size, or 0
 // for an invalid configuration.
 template <class T>
-unsigned step2_grid(const StepParams<T>& p, int block_x) {
-  // uniform force; a whole periodic domain, or the owned rows of a strip with two ghost rows
-  // on each side; tile rows >= 1; a plane addressable with 32 bits and holding the discard
-  // slots
+unsigned step2_grid(const StepParams<T>& p, int block_x, bool edges = false) {
+  // a whole periodic-y domain, or the owned rows of a strip with two ghost rows on each side
+  // (uniform force, periodic x); tile rows >= 1; a plane addressable with 32 bits and holding
+  // the discard slots. `edges`: open x or a force field allowed (whole domains only).
   const bool whole = p.wrap_y && p.row0 == 0 && p.nrows == p.ny;
   const bool strip = !p.wrap_y && p.row0 >= 2 && p.nrows >= 1 && p.row0 + p.nrows <= p.ny - 2;
-  if (!(whole || strip) || p.force || p.rows_per_tile < 1 ||
+  if (!(whole || strip) || (p.force && !(edges && whole)) || p.rows_per_tile < 1 ||
       block_x < 64 || block_x > 512 || block_x % 64 ||
       p.tiles_x != (p.nx + block_x - 1) / block_x ||
       static_cast<uint64_t>(p.N) * sizeof(T) >= (uint64_t{1} << 32) ||
@@ -609,19 +716,20 @@ size_t batch_step2_lds_bytes(int block_x, size_t elem) {
 }
 
 template <class T>
-hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream) {
-  const unsigned grid = step2_grid(p, block_x);
-  if (!grid) return hipErrorInvalidValue;
+hipError_t launch_step2(const StepParams<T>& p, int block_x, hipStream_t stream, bool open_x) {
+  const unsigned grid = step2_grid(p, block_x, true);
+  // open x: a whole domain at least 3 columns wide (inlet, outlet and a column between)
+  if (!grid || (open_x && (!p.wrap_y || p.nx < 3))) return hipErrorInvalidValue;
   const size_t lds = step2_lds_bytes(block_x, sizeof(T));
   const bool one = p.one_rsrc && static_cast<uint64_t>(kQ) * p.N * sizeof(T) < (uint64_t{1} << 32);
   if (one)
-    launch_planes<T, PlanesOne>(p, grid, block_x, lds, stream);
+    launch_planes<T, PlanesOne>(p, open_x, grid, block_x, lds, stream);
   else
-    launch_planes<T, Planes>(p, grid, block_x, lds, stream);
+    launch_planes<T, Planes>(p, open_x, grid, block_x, lds, stream);
   return hipGetLastError();
 }
-template hipError_t launch_step2<float>(const StepParams<float>&, int, hipStream_t);
-template hipError_t launch_step2<double>(const StepParams<double>&, int, hipStream_t);
+template hipError_t launch_step2<float>(const StepParams<float>&, int, hipStream_t, bool);
+template hipError_t launch_step2<double>(const StepParams<double>&, int, hipStream_t, bool);
 
 template <class T>
 hipError_t launch_step2_halo(const StepParams<T>& p, const HaloOut<T>& h, int block_x,

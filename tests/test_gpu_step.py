@@ -354,12 +354,49 @@ def test_two_step_path_is_the_default_where_it_applies(lbm):
     s = lbm.Solver(256, 64, 0.8, 1e-5, 0.0, lbm.make_channel(256, 64), "float")
     assert s.steps_per_launch()["in_use"]
     s.set_inflow_outflow(0.01)
-    assert not s.steps_per_launch()["in_use"]   # open x: single steps
+    assert s.steps_per_launch()["in_use"]   # open x: two steps too
     s2 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "fp16")
     assert s2.steps_per_launch()["in_use"]  # FP16 storage too
+    s2.set_inflow_outflow(0.01)
+    assert not s2.steps_per_launch()["in_use"]  # but FP16 with open x: single steps
     s3 = lbm.Solver(256, 64, 0.8, 0.0, 0.0, lbm.make_channel(256, 64), "double")
     s3.set_force_field(np.zeros(2 * 256 * 64))
-    assert not s3.steps_per_launch()["in_use"]  # per-cell force field: single steps
+    assert s3.steps_per_launch()["in_use"]  # per-cell force field: two steps
+
+
+@pytest.mark.parametrize("prec", ("float", "double"))
+@pytest.mark.parametrize("nx,block,field", [(513, 512, False), (130, 64, True), (513, 256, True),
+                                             (200, 128, False)])
+def test_two_step_open_x_and_force_field_match_single_steps(lbm, prec, nx, block, field):
+    """Open x (inlet column 0, outlet column nx - 1) and a per-cell force field on the two-step
+    walk: tiles whose last one holds the outlet column alone (nx = 513 with 512- or 256-column
+    tiles: the outlet is a halo column of the tile before it), obstacles touching the inlet and
+    the outlet (outlet bounce-back bits), walls; 9 steps (4 two-step launches and one single
+    step) bit for bit equal to 9 single steps."""
+    ny = 70
+    solid = lbm.make_channel(nx, ny).reshape(ny, nx)
+    solid[20:30, 0:6] = 1              # on the inlet
+    solid[40:52, nx - 4:nx] = 1        # on the outlet
+    solid[10:18, nx // 2:nx // 2 + 9] = 1
+    solid = solid.reshape(-1)
+    dt = np.float32 if prec == "float" else np.float64
+    if prec == "double":  # its LDS ring holds twice the bytes: at most 256 columns
+        block = min(block, 256)
+    rng = np.random.default_rng(nx + block)
+    F = (rng.standard_normal(2 * nx * ny) * 1e-5).astype(dt)
+    out = []
+    for spl in (2, 1):
+        s = lbm.Solver(nx, ny, 0.7, 2e-6, 0.0, solid, prec)
+        s.set_inflow_outflow(0.03)
+        if field:
+            s.set_force_field(F)
+        s.set_steps_per_launch(spl, block, 16)
+        assert s.steps_per_launch()["in_use"] == (spl == 2)
+        s.init_equilibrium(1.0, 0.03, 0.0)
+        s.run(9)
+        out.append(s.populations())
+        s.close()
+    assert_bitwise(out[0], out[1], f"open x {prec} nx={nx} block={block} field={field}")
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
