@@ -10,15 +10,18 @@
 //     fma(B_i, c_i.u, A_i - C_k u.F); the constants come precomputed with the launch
 //     (half_cell.hpp, shared with the two-step kernel).
 //   * Stores round stochastically to binary16 with the hardware conversion v_cvt_sr_f16_f32
-//     (its rounding threshold comes from the high bits of a 32-bit random operand): a per-cell,
-//     per-step hash, advanced by the golden-ratio increment for each of the nine values.
+//     (its rounding threshold comes from the high bits of a 32-bit random operand): a hash of
+//     the cell and the step pair (steps 2p and 2p + 1 read it as is and rotated by 16 bits),
+//     advanced by the golden-ratio increment for each of the nine values.
 //     Round-to-nearest would be biased the same way in every cell of a uniform region and would
 //     stall the slow diffusive modes once their per-step change drops below half an ulp; the
 //     unbiased rounding keeps the expected update exact.
 //   * The nine roundings of a cell are chained so that its mass and momentum carry only the
 //     noise of the finest values: the axis directions are rounded first and their momentum
-//     error is taken off the diagonals (a mass-neutral shift), then the mass error of the eight
-//     moving directions goes into the rest population. Conserved moments integrate rounding
+//     error is taken off the diagonals (a mass-neutral shift), then the rest population is set
+//     to the cell's pre-collision mass less the eight rounded moving values (BGK and the Guo
+//     source conserve the mass), so the stored mass changes only by that value's rounding.
+//     Conserved moments integrate rounding
 //     noise without damping (total mass, and the x-staggered momentum mode u ~ (-1)^(x+t),
 //     which D2Q9 streaming and BGK leave invariant for any tau), so this cuts the mass drift
 //     by two orders of magnitude and the staggered-mode noise about threefold.
@@ -68,15 +71,16 @@ __global__ void __launch_bounds__(1024) step_fp16_kernel(const StepParamsH p) {
     }
     if (active) {
       const uint32_t cell = static_cast<uint32_t>(y) * static_cast<uint32_t>(nx) + xc;
+      float mass;
       if constexpr (kField) {
         HalfCollide c2;
         c2.setup(p.omega, p.fp, p.scale, p.inv_scale, p.force[cell], p.force[p.N + cell]);
-        fast_collide(c2, v);
+        mass = fast_collide(c2, v);
       } else {
-        fast_collide(p.col, v);
+        mass = fast_collide(p.col, v);
       }
       unsigned short h[kQ];
-      round_cell(v, hash_cell(cell, p.seed), p.col.unit, h);
+      round_cell(v, mass, step_random(hash_cell(cell, p.seed), p.shift), p.col.unit, h);
 #pragma unroll
       for (int i = 0; i < kQ; ++i) gn.store_bits(i, cols.c, rows.c, h[i]);
     }

@@ -18,6 +18,14 @@ __device__ __forceinline__ uint32_t hash_cell(uint32_t c, uint32_t seed) {
   return h;
 }
 
+// The random word of a cell for one step. Steps 2p and 2p + 1 share the hash of (cell, seed of
+// pair p): the even step reads it as is, the odd step rotated by 16 bits (shift = 16), so the
+// rounding thresholds (bits 19-31 of each word, see to_half_sr) of the two steps come from
+// disjoint bits of the hash. The two-step kernel hashes a cell once per launch.
+__device__ __forceinline__ uint32_t step_random(uint32_t h, uint32_t shift) {
+  return __builtin_amdgcn_alignbit(h, h, shift);
+}
+
 // fp32 (scaled units) -> binary16 bits, stochastic rounding driven by the high bits of r
 // (bits 19-31 decide, measured). The instruction writes the low half of its destination and
 // leaves the high half as it was. Through the builtin the compiler clears a register for that
@@ -38,28 +46,39 @@ __device__ __forceinline__ float half_value(unsigned short h) {
 }
 
 // Moment-compensated stochastic rounding of one cell's post-collision values (scaled units).
-// Rounding errors e = half(v) - v are exact in fp32 (v and its rounding are within a factor 2);
-// each is one mixed-precision fma(half(v), one, -v) with one = 1 (see HalfCollide::unit).
-__device__ __forceinline__ void round_cell(const float (&v)[kQ], uint32_t r, float one,
-                                           unsigned short (&h)[kQ]) {
+// The axis directions are rounded first; their momentum error (ex, ey) is taken off the
+// diagonals by the mass-neutral shift -(ex c_x + ey c_y) / 4 before those are rounded; the rest
+// population is then set from the cell's pre-collision mass (BGK and the Guo source conserve
+// it) less the eight rounded moving values, so the stored mass changes by the rounding of that
+// one value only. v[0] is not read. Rounding errors e = half(v) - v are exact in fp32 (v and its
+// rounding are within a factor 2); each is one mixed-precision fma(half(v), one, -v) with
+// one = 1 (see HalfCollide::unit), and so are the sums of rounded values.
+__device__ __forceinline__ void round_cell(const float (&v)[kQ], float mass, uint32_t r,
+                                           float one, unsigned short (&h)[kQ]) {
   uint32_t rnd[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) rnd[i] = r + static_cast<uint32_t>(i) * 0x9E3779B9u;
-  float e[kQ];
+  float e[5];
 #pragma unroll
   for (int i = 1; i <= 4; ++i) {
     h[i] = to_half_sr(v[i], rnd[i]);
     e[i] = __builtin_fmaf(half_value(h[i]), one, -v[i]);
   }
   const float ex = e[1] - e[3], ey = e[2] - e[4];
-  float em = (e[1] + e[3]) + (e[2] + e[4]);
-#pragma unroll
-  for (int i = 5; i < kQ; ++i) {  // sum_d c_d (-(ex c_dx + ey c_dy) / 4) = -(ex, ey), sum_d = 0
-    const float t = v[i] - 0.25f * (kCx[i] * ex + kCy[i] * ey);
-    h[i] = to_half_sr(t, rnd[i]);
-    em += __builtin_fmaf(half_value(h[i]), one, -v[i]);
-  }
-  h[0] = to_half_sr(v[0] - em, rnd[0]);
+  // sum_d c_d (-(ex c_dx + ey c_dy) / 4) = -(ex, ey) over the diagonals d, and sum_d of the
+  // shift = 0: directions 5 (1, 1), 6 (-1, 1), 7 (-1, -1), 8 (1, -1)
+  const float sp = ex + ey, sm = ey - ex;
+  h[5] = to_half_sr(__builtin_fmaf(-0.25f, sp, v[5]), rnd[5]);
+  h[6] = to_half_sr(__builtin_fmaf(-0.25f, sm, v[6]), rnd[6]);
+  h[7] = to_half_sr(__builtin_fmaf(0.25f, sp, v[7]), rnd[7]);
+  h[8] = to_half_sr(__builtin_fmaf(0.25f, sm, v[8]), rnd[8]);
+  // mass - sum_{i >= 1} half(h_i), in independent partial sums
+  const float a = __builtin_fmaf(half_value(h[1]), -one, mass);
+  const float b = __builtin_fmaf(half_value(h[2]), one, half_value(h[3]));
+  const float c = __builtin_fmaf(half_value(h[4]), one, half_value(h[5]));
+  const float d = __builtin_fmaf(half_value(h[6]), one, half_value(h[7]));
+  const float rest = __builtin_fmaf(half_value(h[8]), -one, a - ((b + c) + d));
+  h[0] = to_half_sr(rest, rnd[0]);
 }
 
 // Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
@@ -70,8 +89,10 @@ __device__ __forceinline__ constexpr int wclass(int i) { return i == 0 ? 0 : (i 
 // is formed per direction, so its two terms cancel at their own small magnitude. Every fused
 // multiply-add is written out (no contraction left to the compiler), so the one-step and
 // two-step FP16 kernels perform the same operations on a cell; sums with an operand that may
-// come straight from binary16 storage are fma(a, unit, b).
-__device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ]) {
+// come straight from binary16 storage are fma(a, unit, b). Returns the cell's mass S drho (the
+// sum of its nine pulled values). v[0] is left as it was: round_cell sets the rest population
+// from the mass.
+__device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[kQ]) {
   const float one = c.unit;
   auto add = [one](float a, float b) { return __builtin_fmaf(a, one, b); };
   auto sub = [one](float a, float b) { return __builtin_fmaf(a, one, -b); };
@@ -80,20 +101,20 @@ __device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ
   const float mx = (sub(v[1], v[3]) + d57) + d86;
   const float my = (sub(v[2], v[4]) + d57) - d86;
   const float rho = __builtin_fmaf(m0, c.invS, 1.0f);
-  const float ir = __builtin_amdgcn_rcpf(rho);
-  const float ux = __builtin_fmaf(mx, c.invS, c.hfx) * ir;
-  const float uy = __builtin_fmaf(my, c.invS, c.hfy) * ir;
+  // u = (m / S + F / 2) / rho = (m + S F / 2) (1 / (S rho)): S a power of two, so the same
+  // value as fma(m, 1 / S, F / 2) / rho, with one scalar operand per instruction
+  const float irS = __builtin_amdgcn_rcpf(rho) * c.invS;
+  const float ux = (mx + c.hfxS) * irS;
+  const float uy = (my + c.hfyS) * irS;
   const float usq15 = 1.5f * __builtin_fmaf(ux, ux, uy * uy);
   const float uF = __builtin_fmaf(ux, c.fx, uy * c.fy);
-  float ac[3], b[3];  // per class: omega w S drho - C u.F, b_k
+  float ac[3], b[3];  // per class: omega w S drho - C u.F, b_k (class 0 unused: no rest update)
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 1; k < 3; ++k) {
     ac[k] = __builtin_fmaf(c.ow[k], m0, -(c.C[k] * uF));
     b[k] = c.owS[k] * rho;
   }
   const float b3[3] = {0.0f, c.owS3[1] * rho, c.owS3[2] * rho};
-  // rest: cu = 0, A_0 = B_0 = 0
-  v[0] = __builtin_fmaf(c.om1, v[0], __builtin_fmaf(b[0], -usq15, ac[0]));
   // pairs (direction, opposite) with the direction's cu: (1, 3) ux, (2, 4) uy, (5, 7) ux + uy,
   // (6, 8) uy - ux
   const float spp = ux + uy, smp = uy - ux;
@@ -109,6 +130,7 @@ __device__ __forceinline__ void fast_collide(const HalfCollide& c, float (&v)[kQ
     v[i] = __builtin_fmaf(c.om1, v[i], even + odd);
     v[o] = __builtin_fmaf(c.om1, v[o], even - odd);
   }
+  return m0;
 }
 
 }  // namespace lbmhip

@@ -57,7 +57,7 @@ struct StepParams {
 // mixed-precision fused multiply-add instead of being converted by an instruction of its own
 // (the compiler turns a literal fma(a, 1, b) back into an addition).
 struct HalfCollide {
-  float om1, invS, hfx, hfy, fx, fy, unit;
+  float om1, invS, hfxS, hfyS, fx, fy, unit;  // hf.S = half the force in scaled units
   float ow[3], owS[3], owS3[3];  // omega w_k, omega w_k S, 3 omega w_k S
   float A[9], B[9], C[3];
 
@@ -66,8 +66,8 @@ struct HalfCollide {
     invS = inv_s;
     fx = fx_;
     fy = fy_;
-    hfx = 0.5f * fx_;
-    hfy = 0.5f * fy_;
+    hfxS = 0.5f * fx_ * S;  // S a power of two: exact
+    hfyS = 0.5f * fy_ * S;
     unit = 1.0f;
     for (int i = 0; i < kQ; ++i) {
       const float wi = weight<float>(i);
@@ -102,7 +102,8 @@ struct StepParamsH {
   float omega, fp, fx, fy;
   float scale, inv_scale;
   float eq_in[9];  // inlet equilibrium, already scaled
-  uint32_t seed;   // stochastic-rounding stream of this step
+  uint32_t seed;   // stochastic-rounding seed of this step's pair (steps 2p, 2p + 1)
+  uint32_t shift;  // 0 on the pair's even step, 16 on its odd step (step_random)
   int xcd_bands;   // tile order, as StepParams::xcd_bands
   HalfCollide col; // uniform-force collision constants
 };
@@ -217,11 +218,11 @@ size_t batch_step2_lds_bytes(int block_x, size_t elem);
 template <class T>
 int step2_tile_rows(const StepParams<T>& p, int block_x, int min_waves = 1);
 
-// FP16 storage, two steps per launch (step2.hip): the conditions of launch_step2; p.seed is the
-// stochastic-rounding seed of the first step, seed2 of the second. Bit-identical to two
-// launch_step_fp16 steps. LDS per workgroup: step2_lds_bytes(block_x, 2) (the ring holds the
+// FP16 storage, two steps per launch (step2.hip): the conditions of launch_step2, and the launch
+// starts on an even step (p.shift == 0; p.seed is the seed of the step pair). Bit-identical to
+// two launch_step_fp16 steps. LDS per workgroup: step2_lds_bytes(block_x, 2) (the ring holds the
 // step-1 values in binary16).
-hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream);
+hipError_t launch_step2_fp16(const StepParamsH& p, int block_x, hipStream_t stream);
 int step2_tile_rows_fp16(const StepParamsH& p, int block_x);
 
 // ---- strips (one GPU's share of the v4 decomposition) ----------------------------------------

@@ -108,7 +108,8 @@ StepParamsH make_step_params_h(const SolverState& s) {
   float eq[kQ] = {};
   if (s.open_x) equilibrium_state<float>(1.0f, static_cast<float>(s.u_in), 0.0f, eq);
   for (int i = 0; i < kQ; ++i) p.eq_in[i] = eq[i] * p.scale;
-  p.seed = fp16_seed(s.step_count);
+  p.seed = fp16_seed(s.step_count >> 1);
+  p.shift = (s.step_count & 1u) ? 16u : 0u;
   p.xcd_bands = s.tile_order_for(s.rows);
   p.col.setup(p.omega, p.fp, p.scale, p.inv_scale, p.fx, p.fy);
   return p;
@@ -116,8 +117,8 @@ StepParamsH make_step_params_h(const SolverState& s) {
 
 }  // namespace
 
-// Stochastic-rounding seed of FP16 step number `step`.
-uint32_t fp16_seed(uint32_t step) { return step * 0x9E3779B9u + 0x7F4A7C15u; }
+// Stochastic-rounding seed of FP16 step pair `pair` (steps 2 pair and 2 pair + 1).
+uint32_t fp16_seed(uint32_t pair) { return pair * 0x9E3779B9u + 0x7F4A7C15u; }
 
 namespace {
 
@@ -369,6 +370,11 @@ int SolverState::two_step_rows() const {
 }
 
 int SolverState::advance(int n) {
+  // an FP16 two-step launch starts on an even step (its two steps share one hash per cell)
+  if (prec == LBM_FP16 && (step_count & 1u) && n >= 2 && two_step_eligible()) {
+    LBM_TRY(step());
+    --n;
+  }
   if (n >= 2 && two_step_eligible()) {
     hipError_t e = hipSuccess;
     const int rows_tile = two_step_rows();
@@ -376,11 +382,11 @@ int SolverState::advance(int n) {
     const int order = two_step_order == 2 && ((ny + rows_tile - 1) / rows_tile) % 8 ? 0 : two_step_order;
     for (; n >= 2 && e == hipSuccess; n -= 2) {
       if (prec == LBM_FP16) {
-        StepParamsH p = make_step_params_h(*this);  // seed of step step_count
+        StepParamsH p = make_step_params_h(*this);  // seed of the pair of step step_count
         p.rows_per_tile = rows_tile;
         p.tiles_x = (nx + block2_x - 1) / block2_x;
         p.xcd_bands = order;
-        e = launch_step2_fp16(p, fp16_seed(step_count + 1), block2_x, stream);
+        e = launch_step2_fp16(p, block2_x, stream);
       } else if (prec == LBM_FLOAT) {
         StepParams<float> p = make_step_params<float>(*this);
         p.rows_per_tile = rows_tile;

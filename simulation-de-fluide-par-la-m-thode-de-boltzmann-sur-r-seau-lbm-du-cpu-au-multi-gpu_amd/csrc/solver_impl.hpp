@@ -93,7 +93,7 @@ struct SolverState {
 // same offset: the nine streams of a cell then spread over DRAM channels and banks (measured
 // 1-2 % faster at 8192^2 on MI355X). Smaller grids keep S = N.
 size_t plane_stride(size_t N, size_t elem_bytes);
-uint32_t fp16_seed(uint32_t step);  // stochastic-rounding seed of FP16 step number `step`
+uint32_t fp16_seed(uint32_t pair);  // stochastic-rounding seed of FP16 step pair `pair`
 
 int copy_ceiling(size_t bytes, int reps, double* copy_gbs, double* memcpy_gbs,
                  double* reference_gbs, double* nine_gbs);

@@ -16,7 +16,7 @@
 //      thread's own registers (rows r - 2, r - 1, r), the others from the ring (columns x - 1,
 //      x + 1), bounce-back from the thread's own R_{t+1} values of row r - 1; and stores it.
 // Step 1 therefore runs on rows y0 - 1 .. y1 (one extra row on each side of the tile) and the
-// two columns just outside the tile (x0 - 1 and x0 + W) are computed for 32 rows at a time
+// two columns just outside the tile (x0 - 1 and x0 + W) are computed for 16 rows at a time
 // (only the three populations the tile pulls from each of them are kept). Each ring row holds
 // W + 2 columns: column 0 is the left halo column, 1 .. W the tile, W + 1 the right halo, so a
 // lane reads its x - 1 and x + 1 neighbours at fixed offsets, the tile edges included. One
@@ -106,7 +106,8 @@ struct Step2View {
 
 // Cell updates of the walk. Stored type S, computed type C, written word Word; step1() turns a
 // cell's pulled values into its state one step later as the next step will read it, step2()
-// into the words the store writes. `cell` is the cell's index y * nx + x.
+// into the words the store writes. `cell` is the cell's index y * nx + x; step1() returns a word
+// that the walk hands to step2() of the same cell (FP16: the cell's hash for the step pair).
 // What an open-x domain or a per-cell force field adds to a walk (float / double cells only):
 // the inlet equilibrium and the force planes (fx at force, fy at force + N).
 template <class T>
@@ -124,8 +125,15 @@ struct PlainCells {
   // step 2's bounce-back reads the step-1 row above from the LDS ring (6 fewer values in
   // registers: fp32 117 -> 111 VGPRs, fp64 195 -> 183)
   static constexpr bool kRowInRegisters = false;
+  // double: the next row's pulls are issued after step 1 (before the barrier) rather than
+  // before it, so they are not in flight across step 1's collision: 8192^2 fp64 70.8k -> 71.3k
+  // MLUPS, 8192 x 6144 67.5k -> 70.1k; float and FP16 measured no different either way
+  static constexpr bool kPullLate = sizeof(T) == 8;
   T omega, fp, fx, fy;
-  __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t) const { collide(v, fx, fy, omega, fp); }
+  __device__ __forceinline__ uint32_t step1(Ring (&v)[kQ], uint32_t) const {
+    collide(v, fx, fy, omega, fp);
+    return 0u;
+  }
   __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t, Word (&out)[kQ]) const {
     collide(v, fx, fy, omega, fp);
 #pragma unroll
@@ -144,9 +152,10 @@ struct PlainCells {
   }
 };
 
-// FP16 storage: the collision and the stochastic rounding of step_fp16_kernel, with the seed of
-// each of the two steps, so the result is bit-identical to two one-step launches; step 2 reads
-// the rounded step-1 values, as the second launch would.
+// FP16 storage: the collision and the stochastic rounding of step_fp16_kernel, for a launch
+// that starts on an even step 2p: step 1 rounds with the cell's hash of pair p as is, step 2 with
+// the same hash rotated (step_random), so the result is bit-identical to two one-step launches;
+// step 2 reads the rounded step-1 values, as the second launch would.
 // The walk carries binary16 numbers (the stored values and the rounded step-1 values), in
 // registers and in the LDS ring; they are converted only inside the cell update, where every
 // use is a (mixed-precision) fused multiply-add that reads the 16-bit value directly.
@@ -157,24 +166,27 @@ struct HalfCells {
   // the FP16 walk keeps the whole step-1 row in registers (87 VGPRs; reading it from the ring
   // measured 109)
   static constexpr bool kRowInRegisters = true;
+  static constexpr bool kPullLate = false;
   HalfCollide col;
-  uint32_t seed1, seed2;
-  __device__ __forceinline__ void step1(Ring (&v)[kQ], uint32_t cell) const {
+  uint32_t seed;  // seed of the step pair
+  __device__ __forceinline__ uint32_t step1(Ring (&v)[kQ], uint32_t cell) const {
     float f[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = static_cast<float>(v[i]);
-    fast_collide(col, f);
-    unsigned short h[kQ];
-    round_cell(f, hash_cell(cell, seed1), col.unit, h);
+    const float mass = fast_collide(col, f);
+    const uint32_t h = hash_cell(cell, seed);
+    unsigned short o[kQ];
+    round_cell(f, mass, h, col.unit, o);
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) v[i] = __builtin_bit_cast(_Float16, h[i]);
+    for (int i = 0; i < kQ; ++i) v[i] = __builtin_bit_cast(_Float16, o[i]);
+    return h;
   }
-  __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t cell, Word (&out)[kQ]) const {
+  __device__ __forceinline__ void step2(Ring (&v)[kQ], uint32_t h, Word (&out)[kQ]) const {
     float f[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = static_cast<float>(v[i]);
-    fast_collide(col, f);
-    round_cell(f, hash_cell(cell, seed2), col.unit, out);
+    const float mass = fast_collide(col, f);
+    round_cell(f, mass, step_random(h, 16u), col.unit, out);
   }
   template <int, class PS>
   __device__ __forceinline__ static void store(const PS& gn, int i, unsigned vo, unsigned so, Word w) {
@@ -225,7 +237,7 @@ __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, con
       const S cfy = Stored<S>::load(make_rsrc(ex->force + p.N), cols.c + rows.c, 0u);
       cells.step1(v, cfx, cfy);
     } else {
-      cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(p.nx) + static_cast<uint32_t>(c));
+      (void)cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(p.nx) + static_cast<uint32_t>(c));
     }
   }
 }
@@ -367,6 +379,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
   T prev2 = T(0);       // plane 2 of R_{t+1} at (c, r - 2)
   uint32_t prev_link = kLinkSolid;
+  uint32_t prev_carry = 0;  // what step 1 at (c, r - 1) handed to its step 2
   T cur[kQ];            // raw pulls of the current step-1 row
   auto pull = [&](const Rows& rw, T (&out)[kQ]) {
 #pragma unroll
@@ -406,7 +419,8 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   };
 
   // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
-  // are issued first and nothing between them and their use branches around a vector memory
+  // are issued first (P::kPullLate: after step 1, before the barrier) and nothing between them
+  // and their use branches around a vector memory
   // operation (the bounce-back loads complete inside their own branch, solid cells store to
   // the discard slots): one wait counter covers loads and stores, and a branch around either
   // would make the wait for the pulls a wait for every store in flight.
@@ -417,12 +431,15 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
     const int rn = k + 1 < nr ? wrap_y(y0 + k) : r;
     const Rows rows_n = uniform_rows_of(rn, nx, ny, B);
-    pull(rows_n, nxt);
     S fnx = S(0), fny = S(0);
-    if constexpr (kField) {
-      fnx = Stored<S>::load(fxr, cols.c, rows_n.c);
-      fny = Stored<S>::load(fyr, cols.c, rows_n.c);
-    }
+    auto issue_next = [&]() __attribute__((always_inline)) {
+      pull(rows_n, nxt);
+      if constexpr (kField) {
+        fnx = Stored<S>::load(fxr, cols.c, rows_n.c);
+        fny = Stored<S>::load(fyr, cols.c, rows_n.c);
+      }
+    };
+    if constexpr (!P::kPullLate) issue_next();
     // step 1 at (c, r), exactly as update_column: bounce-back corrections, then the collision
     // (of a solid cell too: its values are never read, its fluid neighbours bounce back)
     T v[kQ];
@@ -436,7 +453,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
         for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? bown[i] : v[i];
       }
       // the next row's (its loads complete by the next row's wait for its pulls)
-      if (k + 1 < nr) bounce_data(rn, rows_n);
+      if (!P::kPullLate && k + 1 < nr) bounce_data(rn, rows_n);
     } else if (uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6))) {
       link = p.links[static_cast<size_t>(r) * nx + c];
       T own[kQ];
@@ -446,10 +463,11 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       for (int i = 1; i < kQ; ++i) v[i] = (link & (1u << i)) ? own[i] : v[i];
       if constexpr (kOpen) apply_open_edges<S>(link, g, rows, cols, ex->eq_in, v);
     }
+    uint32_t carry = 0;
     if constexpr (kField)
       cells.step1(v, f1x, f1y);
     else
-      cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c));
+      carry = cells.step1(v, static_cast<uint32_t>(r) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c));
     T* sl = ring + slot * kRingPlanes * WP;
 #pragma unroll
     for (int i : {1, 3, 5, 6, 7, 8}) sl[ring_index(i) * WP + tt + 1] = v[i];
@@ -471,6 +489,12 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
         const T* h = halo + (kHaloRows + hk) * 3;
 #pragma unroll
         for (int q = 0; q < 3; ++q) sl[ring_index(kRightPlanes[q]) * WP + wv + 1] = h[q];
+      }
+    }
+    if constexpr (P::kPullLate) {
+      issue_next();
+      if constexpr (kAhead) {
+        if (k + 1 < nr) bounce_data(rn, rows_n);
       }
     }
     __syncthreads();
@@ -527,7 +551,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       if constexpr (kField)
         cells.step2(w, f2x, f2y, out);
       else
-        cells.step2(w, static_cast<uint32_t>(q) * static_cast<uint32_t>(nx) + static_cast<uint32_t>(c), out);
+        cells.step2(w, prev_carry, out);
       const unsigned so = static_cast<unsigned>(
           __builtin_amdgcn_readfirstlane(static_cast<unsigned>(q * nx) * B));
       const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
@@ -573,6 +597,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       cur[i] = nxt[i];
     }
     prev_link = link;
+    prev_carry = carry;
     if constexpr (kField) {
       f2x = f1x;
       f2y = f1y;
@@ -604,16 +629,16 @@ __global__ void __launch_bounds__(512) step2_kernel(const StepParams<T> p, const
       p.row0 + p.nrows, smem, static_cast<int>(blockDim.x), static_cast<int>(threadIdx.x), &h, &ex);
 }
 
-// FP16 storage, two steps: seeds of steps t (p.seed) and t + 1 (seed2).
+// FP16 storage, two steps from an even step: p.seed is the seed of the step pair.
 template <class PS>
-__global__ void __launch_bounds__(512) step2_fp16_kernel(const StepParamsH p, const uint32_t seed2) {
+__global__ void __launch_bounds__(512) step2_fp16_kernel(const StepParamsH p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const unsigned tile = tile_in_order(p.xcd_bands, blockIdx.x, gridDim.x,
                                       static_cast<unsigned>(p.tiles_x));
   const int tx = static_cast<int>(tile % static_cast<unsigned>(p.tiles_x));
   const int ty = static_cast<int>(tile / static_cast<unsigned>(p.tiles_x));
   const Step2View<uint16_t> d{p.g, p.gn, p.links, p.segflags, p.N, p.nx, p.ny, p.nseg};
-  const HalfCells cells{p.col, p.seed, seed2};
+  const HalfCells cells{p.col, p.seed};
   step2_walk<HalfCells, PS, kAuxNone>(d, cells, tx * static_cast<int>(blockDim.x),
                                       ty * p.rows_per_tile, p.rows_per_tile, p.ny, smem,
                                       static_cast<int>(blockDim.x), static_cast<int>(threadIdx.x));
@@ -860,10 +885,10 @@ bool fp16_one_resource(const StepParamsH& p) {
 }
 }  // namespace
 
-hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, hipStream_t stream) {
-  // periodic, uniform force, whole domain, tile rows >= 1, planes addressable with 32 bits and
-  // holding the discard slots
-  if (!p.wrap_y || p.force || p.row0 != 0 || p.nrows != p.ny || p.rows_per_tile < 1 ||
+hipError_t launch_step2_fp16(const StepParamsH& p, int block_x, hipStream_t stream) {
+  // from an even step (shift 0), periodic, uniform force, whole domain, tile rows >= 1, planes
+  // addressable with 32 bits and holding the discard slots
+  if (p.shift != 0 || !p.wrap_y || p.force || p.row0 != 0 || p.nrows != p.ny || p.rows_per_tile < 1 ||
       block_x < 64 || block_x > 512 || block_x % 64 ||
       p.tiles_x != (p.nx + block_x - 1) / block_x ||
       static_cast<uint64_t>(p.N) * sizeof(uint16_t) >= (uint64_t{1} << 32) ||
@@ -875,9 +900,9 @@ hipError_t launch_step2_fp16(const StepParamsH& p, uint32_t seed2, int block_x, 
   if (p.xcd_bands == 2 && tiles_y % 8) return hipErrorInvalidValue;
   const unsigned grid = static_cast<unsigned>(p.tiles_x) * tiles_y;
   if (fp16_one_resource(p))
-    step2_fp16_kernel<PlanesOne><<<grid, block_x, lds, stream>>>(p, seed2);
+    step2_fp16_kernel<PlanesOne><<<grid, block_x, lds, stream>>>(p);
   else
-    step2_fp16_kernel<Planes><<<grid, block_x, lds, stream>>>(p, seed2);
+    step2_fp16_kernel<Planes><<<grid, block_x, lds, stream>>>(p);
   return hipGetLastError();
 }
 
