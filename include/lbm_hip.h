@@ -99,7 +99,8 @@ int lbm_get_kernel_options(const lbm_solver* s, int* streaming_stores, int* sing
  * steps (same operations per cell). block_x (64..512, multiple of 64; 0 keeps the current
  * value) and rows set its tile: rows > 0 a fixed height, -1 (the default) the tallest of 64,
  * 48, 32, 24, 16 rows whose tile count fills the GPU's resident workgroup slots evenly, 0 keeps
- * the current setting; tile_order (0, 1, 2 as in lbm_set_kernel_options, -1 keeps; default 1)
+ * the current setting; tile_order (0, 1, 2 as in lbm_set_kernel_options, -1 keeps; default 1,
+ * FP16 storage 2)
  * orders its tiles over the XCDs. lbm_step always advances one step. lbm_get_steps_per_launch
  * reports the height in use and in_use whether the current configuration takes the two-step
  * path. */

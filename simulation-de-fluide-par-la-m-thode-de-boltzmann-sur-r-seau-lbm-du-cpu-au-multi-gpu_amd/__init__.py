@@ -417,7 +417,7 @@ class Solver:
         """1: one step per launch; 2: run()/timed_run() advance pairs of steps in one launch of
         the two-step kernel where it applies (same bits). block_x: tile columns (0 keeps);
         rows: tile rows, -1 automatic (up to 64, filling the GPU evenly), 0 keeps; tile_order:
-        0 round-robin, 1 XCD bands (default), 2 XCD rows, -1 keeps."""
+        0 round-robin, 1 XCD bands (default; FP16 storage: 2), 2 XCD rows, -1 keeps."""
         check(lib().lbm_set_steps_per_launch(self._h, int(steps_per_launch), int(block_x),
                                              int(rows), int(tile_order)))
 

@@ -187,12 +187,15 @@ int SolverState::create(int nx_, int ny_, double tau_, double fx_, double fy_, c
     xcd_bands = prec == LBM_DOUBLE ? 1 : 0;
   }
   // two-step kernel: 512-column tiles in float; 256 in double (its LDS ring doubles; 512
-  // columns exceed the LDS of a workgroup) and in FP16 storage (87 VGPRs: five 4-wave
+  // columns exceed the LDS of a workgroup) and in FP16 storage (88 VGPRs: five 4-wave
   // workgroups per CU). 8192^2: fp32 512 x 64 146.3k MLUPS (256 x 64 128.5k), fp64 256 x 64
-  // 69.6k (128 x 64 63.7k), FP16 256 x 64 210.6k (512 x 64 202.9k); the height comes from
-  // step2_tile_rows().
+  // 69.6k (128 x 64 63.7k), FP16 256 x 64 221.6k (512 x 64 215.5k, 128 x 64 201.9k); the height
+  // comes from step2_tile_rows(). Tile order: XCD bands, except FP16 (XCD rows: its 2-byte
+  // tiles share the most edge lines per byte; 8192^2 230.8k against 229.1k with bands and
+  // 227.2k round-robin).
   block2_x = prec == LBM_FLOAT ? 512 : 256;
   rows2 = -1;
+  two_step_order = prec == LBM_FP16 ? 2 : 1;
   LBM_TRY(parse_block_env(&block_x, &rows));
   LBM_HIP_TRY(hipGetDevice(&device));
   LBM_HIP_TRY(hipStreamCreate(&stream));

@@ -244,10 +244,11 @@ __device__ __forceinline__ void halo_step(const Step2View<typename P::S>& p, con
 
 // Halo-column cells are computed for kHaloChunk step-1 rows at a time (the first chunk also
 // covers the two leading rows), so the LDS a workgroup needs does not grow with its tile height.
-// A multiple of 4 (the main loop's unrolling). 16 rows: the halo columns read their
-// neighbour tile's lines closer to the time that tile reads them (8192^2: fp32 +0.6 %, FP16
-// +0.4 % against 32 rows; 8 rows no better); same bits for any chunk.
-constexpr int kHaloChunk = 16;
+// A multiple of 4 (the main loop's unrolling). The shorter the chunk, the closer to the time
+// the neighbour tile reads its lines the halo columns read them: 8192^2 FP16 229.1k MLUPS with
+// 8 rows against 224.8k with 16 and 224.1k with 4 (fp32 148.6k, 148.5k, 148.3k; 32 rows was
+// 0.6 % slower than 16); same bits for any chunk.
+constexpr int kHaloChunk = 8;
 constexpr int kHaloRows = kHaloChunk + 2;
 
 // The order of a walk over nr step-1 rows: the first halo chunk (kHaloRows rows), start()

@@ -454,10 +454,11 @@ def test_two_step_tile_orders_match_oracle(lbm, oracle_mod, prec):
 @pytest.mark.parametrize("grid,tile", [((53, 31), (64, 7)), ((300, 90), (512, -1)),
                                        ((640, 416), (256, 64)), ((333, 700), (128, 100))])
 def test_fp16_two_step_equals_one_step_launches(lbm, grid, tile):
-    """FP16 storage with two steps per launch: each step's stochastic rounding (its own seed,
-    the cell's own index) applied to the step-1 values before step 2 reads them, so the raw
-    state equals the one-step kernel's bit for bit after even and odd step counts; solid cells
-    keep their values."""
+    """FP16 storage with two steps per launch: each step's stochastic rounding (the hash of the
+    cell and the step pair, read rotated on the odd step) applied to the step-1 values before
+    step 2 reads them, so the raw state equals the one-step kernel's bit for bit after even and
+    odd step counts (run(3) from step 3 takes one single step, then a two-step launch from an
+    even step); solid cells keep their values."""
     nx, ny = grid
     solid = lbm.make_channel(nx, ny).reshape(ny, nx)
     solid[ny // 3: ny // 3 + 9, min(nx - 12, 40): min(nx - 12, 40) + 7] = 1
