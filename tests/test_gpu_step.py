@@ -201,6 +201,32 @@ def test_channel_8192_columns_match_oracle_narrow_channel(lbm, oracle_mod):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
+def test_block_2048_1000_steps_match_oracle(lbm, oracle_mod, prec):
+    """The plain method's throughput instance at its real size: 2048 x 2048, walls at y = 0 and
+    ny - 1, the 8 x 8 block placed by seed 0, tau 0.8, F = (1e-5, 2e-6), rest init, 1000 steps
+    on the default path (two steps per launch), every population bit for bit against the
+    reference's own v1 OpenMP solver compiled from its sources (oracle/_ref), or, where that was
+    not built, the oracle's OpenMP pull port (bit-identical to the v0 restatement,
+    tests/test_oracle.py)."""
+    n, steps = 2048, 1000
+    solid = lbm.seeded_block_mask(0, n, n)
+    s = lbm.Solver(n, n, 0.8, 1e-5, 2e-6, solid, prec)
+    assert s.steps_per_launch()["in_use"]
+    s.init_equilibrium(1.0, 0.0, 0.0)
+    s.run(steps)
+    got = s.populations()
+    s.close()
+    if oracle_mod.Reference.available(prec):
+        ref = oracle_mod.Reference(prec).run("v1", n, n, 0.8, 1e-5, 2e-6, solid, steps)
+    else:
+        o = oracle_mod.Oracle(n, n, 0.8, 1e-5, 2e-6, solid, prec, "pull")
+        o.init_equilibrium(1.0, 0.0, 0.0)
+        o.run(steps)
+        ref = o.populations()
+    assert_bitwise(got, ref, f"2048^2 block {prec}")
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
 def test_non_multiple_grid_and_tiny_grid(lbm, oracle_mod, prec):
     """Ragged sizes: nx not a multiple of 64, tiles cut in y, a 1-row-thick fluid layer."""
     for nx, ny in ((67, 9), (130, 5), (3, 3), (1, 5)):
