@@ -354,8 +354,8 @@ bool SolverState::two_step_eligible() const {
   // open x and force fields: float and double (FP16 storage keeps one step per launch there)
   if (steps_per_launch != 2 || ((open_x || force) && prec == LBM_FP16)) return false;
   if (S * storage_size(prec) >= (size_t(1) << 32)) return false;
-  // the LDS ring holds computed values: fp32 for FP16 storage
-  return step2_lds_bytes(block2_x, prec == LBM_DOUBLE ? 8 : 4) <= 64 * 1024;
+  // the LDS ring holds the walk's values: binary16 for FP16 storage
+  return step2_lds_bytes(block2_x, prec == LBM_DOUBLE ? 8 : (prec == LBM_FP16 ? 2 : 4)) <= 64 * 1024;
 }
 
 int SolverState::two_step_rows() const {
