@@ -478,7 +478,8 @@ def test_two_step_tile_orders_match_oracle(lbm, oracle_mod, prec):
 
 
 @pytest.mark.parametrize("grid,tile", [((53, 31), (64, 7)), ((300, 90), (512, -1)),
-                                       ((640, 416), (256, 64)), ((333, 700), (128, 100))])
+                                       ((640, 416), (256, 64)), ((333, 700), (128, 100)),
+                                       ((640, 512), (256, 64))])
 def test_fp16_two_step_equals_one_step_launches(lbm, grid, tile):
     """FP16 storage with two steps per launch: each step's stochastic rounding (the hash of the
     cell and the step pair, read rotated on the odd step) applied to the step-1 values before
@@ -497,6 +498,8 @@ def test_fp16_two_step_equals_one_step_launches(lbm, grid, tile):
         s = lbm.Solver(nx, ny, 0.7, 2e-5, -5e-6, solid, "fp16")
         s.set_steps_per_launch(spl, *tile)
         assert s.steps_per_launch()["in_use"] == (spl == 2)
+        assert s.steps_per_launch()["tile_order"] == 2  # the FP16 default: XCD rows (8 tile rows
+        # at 640 x 512; the other grids fall back to round-robin)
         s.set_populations(S)
         states = []
         for n in (2, 1, 3, 6):
