@@ -16,7 +16,7 @@
 //      thread's own registers (rows r - 2, r - 1, r), the others from the ring (columns x - 1,
 //      x + 1), bounce-back from the thread's own R_{t+1} values of row r - 1; and stores it.
 // Step 1 therefore runs on rows y0 - 1 .. y1 (one extra row on each side of the tile) and the
-// two columns just outside the tile (x0 - 1 and x0 + W) are computed for 16 rows at a time
+// two columns just outside the tile (x0 - 1 and x0 + W) are computed for 8 rows at a time
 // (only the three populations the tile pulls from each of them are kept). Each ring row holds
 // W + 2 columns: column 0 is the left halo column, 1 .. W the tile, W + 1 the right halo, so a
 // lane reads its x - 1 and x + 1 neighbours at fixed offsets, the tile edges included. One
