@@ -119,7 +119,7 @@ def host_cpu():
     return info
 
 
-# CPU baseline instances (BASELINE.md's CPU-baseline plan): the 2048^2 throughput instance of the
+# CPU baseline instances: the 2048^2 throughput instance of the
 # plain method (walls + seeded 8x8 block, tau 0.8, F = (1e-5, 2e-6)) and the 8192^2 channel
 # (tau 0.6, gx 1e-6), rest init, float. Timed steps per repetition: the protocol's own for all
 # cores; bounded samples for one thread (the full protocol there would take minutes).
@@ -510,8 +510,8 @@ def main():
                     help="CPU baseline: the full protocol steps at one thread too (slow)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: every box here has a single MI355X and nothing in "
-              f"this build shards the domain across GPUs (asked for --gpus {args.gpus}).",
+        print(f"bench.py runs on one GPU only: the library computes one GPU's share and does "
+              f"not shard the domain across GPUs (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

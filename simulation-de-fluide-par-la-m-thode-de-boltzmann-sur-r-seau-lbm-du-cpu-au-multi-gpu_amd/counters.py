@@ -18,7 +18,7 @@ summary goes to a Markdown file with a JSON block that bench.py reads:
 `source_tag` hashes the device sources the library is built from; bench.py flags the figures as
 stale when the current sources hash differently.
 
-Run on the GPU box from the repository root (the program itself right after `--`):
+Run on a machine with the GPU, from the repository root (the program itself right after `--`):
 
     python <package>/counters.py run --kernel step2_kernel<float, PlanesOne --cells 67108864 \\
         --updates-per-launch 2 --bytes-per-cell 72 --out profiles/step2_fp32_8192x8192.traffic.md \\
@@ -208,7 +208,7 @@ def main(argv=None) -> None:
                         help="dispatches of the kernel that make one launch of the path")
         sp.add_argument("--out", required=True, help="summary Markdown file")
     r = sub.choices["run"]
-    r.add_argument("--work", default="gpurun_out/counters", help="rocprofv3 output directory")
+    r.add_argument("--work", default="rocprof_counters", help="rocprofv3 output directory")
     r.add_argument("--valu", action="store_true", help="third pass with SQ_INSTS_VALU")
     r.add_argument("--timeout", type=int, default=300)
     r.add_argument("cmd", nargs=argparse.REMAINDER, help="-- program and arguments")

@@ -244,7 +244,7 @@ def test_non_multiple_grid_and_tiny_grid(lbm, oracle_mod, prec):
 @pytest.mark.parametrize("prec", PRECISIONS)
 def test_padded_planes_match_oracle(lbm, oracle_mod, prec):
     """Planes of 1 MiB and more sit a padded stride apart in device memory (a layout choice of
-    this build). 640 x 416 (266,240 cells): walls, a block, an arbitrary state, a per-cell
+    the library). 640 x 416 (266,240 cells): walls, a block, an arbitrary state, a per-cell
     force field, 30 steps, bit for bit against the oracle; the raw pull state survives a
     download/upload round trip; fields and momentum exchange read the padded planes."""
     nx, ny = 640, 416
