@@ -179,16 +179,18 @@ def test_setters_match_oracle_restart(lbm, oracle_mod, prec):
     assert_bitwise(s.populations(), o2.populations(), "setters")
 
 
-def test_channel_8192_columns_match_oracle_narrow_channel(lbm, oracle_mod):
-    """Full-size instance, 8192 x 8192 walled channel, tau 0.6, gx 1e-6, 200 steps (float):
-    the flow is x-invariant, so every column must equal, bit for bit, the oracle's 4 x 8192
-    channel with the same rows. A size-independent property checked at the real size."""
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_channel_8192_columns_match_oracle_narrow_channel(lbm, oracle_mod, prec):
+    """Full-size instance, 8192 x 8192 walled channel, tau 0.6, gx 1e-6, 200 steps (float, and
+    double: the fp64 grid of the same instance): the flow is x-invariant, so every column must
+    equal, bit for bit, the oracle's 4 x 8192 channel with the same rows. A size-independent
+    property checked at the real size."""
     nx = ny = 8192
-    s = lbm.Solver(nx, ny, 0.6, 1e-6, 0.0, lbm.make_channel(nx, ny), "float")
+    s = lbm.Solver(nx, ny, 0.6, 1e-6, 0.0, lbm.make_channel(nx, ny), prec)
     s.init_equilibrium(1.0, 0.0, 0.0)
     s.run(200)
     rho, ux, uy = (a.reshape(ny, nx) for a in s.macroscopic())
-    o = oracle_mod.Oracle(4, ny, 0.6, 1e-6, 0.0, oracle_mod.make_channel(4, ny), "float", "v0")
+    o = oracle_mod.Oracle(4, ny, 0.6, 1e-6, 0.0, oracle_mod.make_channel(4, ny), prec, "v0")
     o.init_equilibrium(1.0, 0.0, 0.0)
     o.run(200)
     r0, u0, v0 = (a.reshape(ny, 4)[:, :1] for a in o.macroscopic())
@@ -198,6 +200,7 @@ def test_channel_8192_columns_match_oracle_narrow_channel(lbm, oracle_mod):
     assert ux[ny // 2, 0] > 0
     m0 = float(nx) * (ny - 2)
     assert abs(s.total_mass() - m0) / m0 < 1e-6
+    s.close()
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
@@ -537,3 +540,31 @@ def test_two_step_degenerate_grids_match_oracle(lbm, oracle_mod, prec, grid):
         done += n
         assert_bitwise(s.populations(), o.populations(), f"{grid} after {done} steps")
     s.close()
+
+
+@pytest.mark.parametrize("grid", [(1, 1), (1, 7), (5, 1), (3, 3), (64, 2), (257, 20), (130, 5)])
+def test_fp16_two_step_degenerate_grids_equal_one_step(lbm, grid):
+    """FP16 storage on the degenerate grids of the test above (a last 256-column tile one cell
+    wide: 257 = 256 + 1): two steps per launch give the raw binary16 state of one step per launch
+    bit for bit, after even and odd step counts (the odd start takes one single step first)."""
+    nx, ny = grid
+    solid = np.zeros(nx * ny, np.uint8)
+    if nx >= 3 and ny >= 3:
+        solid[(ny // 2) * nx + nx // 2] = 1
+    rng = np.random.default_rng(nx * 137 + ny)
+    S = (rng.standard_normal(nx * ny * 9) * 1e-3).astype(np.float32)
+    S.reshape(-1, 9)[solid.astype(bool)] = 0
+    out = []
+    for spl in (2, 1):
+        s = lbm.Solver(nx, ny, 0.9, 1e-5, -2e-6, solid, "fp16")
+        s.set_steps_per_launch(spl)
+        assert s.steps_per_launch()["in_use"] == (spl == 2)
+        s.set_populations(S)
+        states = []
+        for n in (2, 3, 4):
+            s.run(n)
+            states.append(s.get_state())
+        out.append(states)
+        s.close()
+    for k, (a, b) in enumerate(zip(*out)):
+        assert_bitwise(a, b, f"FP16 {grid} after run {k}")
