@@ -221,7 +221,7 @@ int lbm_strip_compute_rows2(lbm_strip* s, int first, int last, int stream);
 int lbm_strip_compute_rows2_to(lbm_strip* s, int first, int last, int stream, lbm_strip* up,
                                lbm_strip* down);
 /* Tile of the two-step launches: block_x columns (64..512 float, ..256 double, multiple of 64;
- * 0 keeps), rows (> 0 fixed, -1 automatic, 0 keeps), tile_order (0..2 as in
+ * 0 keeps; default 256), rows (> 0 fixed, -1 automatic, 0 keeps), tile_order (0..2 as in
  * lbm_set_kernel_options, -1 keeps; default 2). Same bits for every tile. */
 int lbm_strip_set_two_step_tile(lbm_strip* s, int block_x, int rows, int tile_order);
 /* pack_halo: send_up / send_down (lbm_strip_halo_values reals each) from the new state; waits

@@ -108,13 +108,15 @@ __device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[k
   const float uy = (my + c.hfyS) * irS;
   const float usq15 = 1.5f * __builtin_fmaf(ux, ux, uy * uy);
   const float uF = __builtin_fmaf(ux, c.fx, uy * c.fy);
-  float ac[3], b[3];  // per class: omega w S drho - C u.F, b_k (class 0 unused: no rest update)
+  // per moving weight class (0 axis, 1 diagonal; the rest direction is not updated here):
+  // omega w S drho - C u.F, and omega w S rho and 3 omega w S rho
+  float ac[2], b[2], b3[2];
 #pragma unroll
-  for (int k = 1; k < 3; ++k) {
-    ac[k] = __builtin_fmaf(c.ow[k], m0, -(c.C[k] * uF));
-    b[k] = c.owS[k] * rho;
+  for (int k = 0; k < 2; ++k) {
+    ac[k] = __builtin_fmaf(c.ow[k + 1], m0, -(c.C[k + 1] * uF));
+    b[k] = c.owS[k + 1] * rho;
+    b3[k] = c.owS3[k + 1] * rho;
   }
-  const float b3[3] = {0.0f, c.owS3[1] * rho, c.owS3[2] * rho};
   // pairs (direction, opposite) with the direction's cu: (1, 3) ux, (2, 4) uy, (5, 7) ux + uy,
   // (6, 8) uy - ux
   const float spp = ux + uy, smp = uy - ux;
@@ -122,7 +124,7 @@ __device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[k
   const float cus[4] = {ux, uy, spp, smp};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int i = kFirst[j], o = kOpp[i], k = wclass(i);
+    const int i = kFirst[j], o = kOpp[i], k = wclass(i) - 1;
     const float cu = cus[j];
     const float poly = __builtin_fmaf(4.5f * cu, cu, -usq15);
     const float even = __builtin_fmaf(b[k], poly, __builtin_fmaf(c.B[i], cu, ac[k]));

@@ -163,7 +163,7 @@ struct HalfCells {
   using S = uint16_t;
   using Ring = _Float16;
   using Word = unsigned short;
-  // the FP16 walk keeps the whole step-1 row in registers (87 VGPRs; reading it from the ring
+  // the FP16 walk keeps the whole step-1 row in registers (88 VGPRs; reading it from the ring
   // measured 109)
   static constexpr bool kRowInRegisters = true;
   static constexpr bool kPullLate = false;

@@ -34,12 +34,13 @@ struct lbm_strip {
   double tau = 0.8, fx = 0.0, fy = 0.0, u_in = 0.0;
   bool open_x = false;
   int block_x = 512, rows = 1;  // the plain solver's fp32/fp64 defaults (LBM_BLOCK overrides)
-  // two-step tiles: 512 (float) / 256 (double) columns, height chosen by step2_tile_rows()
-  // with at least two full waves of tiles when rows2 = -1 (16 rows at 8192 x 1024); XCD-row
-  // tile order (whole tile rows per XCD; round-robin where the tile rows are not a multiple of
-  // 8). 8192 x 1024 fp32, 32-row tiles: 133.6k MLUPS against 130.6k with XCD bands, 131.1k
-  // round-robin; 16-row tiles 136.2k (XCD rows) and 133.0k (XCD bands)
-  int block2_x = 512, rows2 = -1, order2 = 2;
+  // two-step tiles: 256 columns (float and double), height chosen by step2_tile_rows() with at
+  // least two full waves of tiles when rows2 = -1 (16 rows at 8192 x 1024: 2,048 tiles on four
+  // 256-thread workgroups per CU); XCD-row tile order (whole tile rows per XCD; round-robin
+  // where the tile rows are not a multiple of 8). 8192 x 1024 fp32: 256 x 16 136.1-136.3k
+  // MLUPS (XCD rows), 134.5k (round-robin), 131.5k (XCD bands); 512 x 16 132.4-134.2k (XCD
+  // rows), 131.3k (round-robin); 512 x 32 (one wave) 133.1k
+  int block2_x = 256, rows2 = -1, order2 = 2;
   mutable int rows2_auto = 0;  // the automatic height for the owned rows (0: not yet known)
   std::vector<uint8_t> solid;  // rows() * nx, ghost rows included
   size_t S = 0;                // plane stride (values) of the device buffers
@@ -66,7 +67,7 @@ struct lbm_strip {
     nyl = nyl_;
     prec = prec_;
     ghost = ghost_;
-    block2_x = prec == LBM_DOUBLE ? 256 : 512;
+    block2_x = 256;
     S = plane_stride(cells(), rs());
     if (S * rs() >= (size_t(1) << 32))
       return fail(LBM_EINVAL, "one population plane of the strip must stay below 4 GiB");
