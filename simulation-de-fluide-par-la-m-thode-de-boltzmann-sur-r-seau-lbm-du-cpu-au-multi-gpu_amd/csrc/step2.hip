@@ -51,7 +51,9 @@ constexpr uint32_t kLinkBounce = 0x1feu;  // bits 1..8 of a link code
 
 // Byte offsets of the source rows of step-1 row r (wrapped) and of the source columns of c.
 __device__ __forceinline__ Rows rows_of(int r, int nx, int ny, unsigned B) {
-  const int rm = r == 0 ? ny - 1 : r - 1, rp = r == ny - 1 ? 0 : r + 1;
+  // (r == 0 ? ny : r) - 1 rather than r == 0 ? ny - 1 : r - 1: the compiler matches the latter
+  // to a subtraction with borrow and selects it with vector instructions, uniform r or not
+  const int rm = (r == 0 ? ny : r) - 1, rp = r == ny - 1 ? 0 : r + 1;
   return Rows{static_cast<unsigned>(rm * nx) * B, static_cast<unsigned>(r * nx) * B,
               static_cast<unsigned>(rp * nx) * B};
 }
@@ -367,6 +369,10 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   // column included: they compute its values and store them to its address, the same bits.
   const int tt = t < wv ? t : wv - 1;
   const int c = x0 + tt;
+  // the wave's 64-column segment of the flag array: a tile starts on a multiple of 64 columns,
+  // so every lane of a wave (idle ones included) lies in lane 0's segment; taken from lane 0,
+  // the flag index is scalar arithmetic
+  const int seg = __builtin_amdgcn_readfirstlane(c) >> 6;
   constexpr unsigned B = sizeof(S);
   const unsigned vo = static_cast<unsigned>(c) * B;
   // A solid cell is not written: its step-2 result goes to a discard slot past the plane's
@@ -380,6 +386,9 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   for (int i = 0; i < kQ; ++i) prev[i] = T(0);
   T prev2 = T(0);       // plane 2 of R_{t+1} at (c, r - 2)
   uint32_t prev_link = kLinkSolid;
+  // wave-uniform: the segment flag of row r - 1 (0: every link code of the wave's cells there
+  // is 0, so its step 2 needs no bounce-back, open-edge or solid-cell handling)
+  uint32_t prev_flag = 1;
   uint32_t prev_carry = 0;  // what step 1 at (c, r - 1) handed to its step 2
   T cur[kQ];            // raw pulls of the current step-1 row
   auto pull = [&](const Rows& rw, T (&out)[kQ]) {
@@ -396,7 +405,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     for (int i = 0; i < kQ; ++i) bown[i] = T(0);
   }
   auto bounce_data = [&](const int r, const Rows& rw) __attribute__((always_inline)) {
-    bflag = uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6));
+    bflag = uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + seg);
     if (bflag) {
       blink = p.links[static_cast<size_t>(r) * nx + c];
 #pragma unroll
@@ -446,8 +455,9 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     T v[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) v[i] = cur[i];
-    uint32_t link = 0;
+    uint32_t link = 0, flag;
     if constexpr (kAhead) {
+      flag = bflag;
       link = bflag ? blink : 0u;
       if (bflag) {
 #pragma unroll
@@ -455,7 +465,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       }
       // the next row's (its loads complete by the next row's wait for its pulls)
       if (!P::kPullLate && k + 1 < nr) bounce_data(rn, rows_n);
-    } else if (uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + (c >> 6))) {
+    } else if ((flag = uniform_flag(p.segflags, static_cast<size_t>(r) * p.nseg + seg))) {
       link = p.links[static_cast<size_t>(r) * nx + c];
       T own[kQ];
 #pragma unroll
@@ -515,47 +525,57 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       w[6] = s2[ring_index(6) * WP + tt + 2];
       w[7] = s0[ring_index(7) * WP + tt + 2];
       w[8] = s0[ring_index(8) * WP + tt];
-      const uint32_t bb = prev_link & kLinkBounce;
-      if (__builtin_amdgcn_ballot_w64(bb != 0)) {  // wave-uniform: only rows next to a wall
-        // R_{t+1} at (c, q)
-        T pq[kQ];
+      const int q = y0 - 2 + k;
+      const unsigned so = static_cast<unsigned>(
+          __builtin_amdgcn_readfirstlane(static_cast<unsigned>(q * nx) * B));
+      // the store offset of solid cells (their results go to the discard slots): taken with the
+      // other link rules, or after the collision where bounce-back data is held a row ahead
+      // (kAhead: computed before the collision it cost the batched walk 10 VGPRs, a wave per
+      // SIMD and 9 % of its rate)
+      unsigned vs = vo;
+      if (prev_flag) {  // wave-uniform: a wall, an obstacle or an open edge in the segment
+        const uint32_t bb = prev_link & kLinkBounce;
+        if (__builtin_amdgcn_ballot_w64(bb != 0)) {  // wave-uniform
+          // R_{t+1} at (c, q)
+          T pq[kQ];
 #pragma unroll
-        for (int i = 0; i < kQ; ++i) pq[i] = prev[i];
-        if constexpr (!P::kRowInRegisters) {
+          for (int i = 0; i < kQ; ++i) pq[i] = prev[i];
+          if constexpr (!P::kRowInRegisters) {
 #pragma unroll
-          for (int i : {1, 3, 5, 6, 7, 8}) pq[i] = s1[ring_index(i) * WP + tt + 1];
-        }
+            for (int i : {1, 3, 5, 6, 7, 8}) pq[i] = s1[ring_index(i) * WP + tt + 1];
+          }
 #pragma unroll
-        for (int i = 1; i < kQ; ++i)
-          if (bb & (1u << i)) w[i] = pq[kOpp[i]];
-      }
-      if constexpr (kOpen) {
-        // apply_open_edges on the step-1 values: the inlet equilibrium, and at the outlet what
-        // x - 1 receives, R_{t+1}(i, x, q - c_y) (ring rows q, q - 1, q + 1 at column x) or
-        // R_{t+1}(opp(i), x - 1, q) (ring row q at column x - 1)
-        if (prev_link & kLinkInlet) {
-          w[1] = ex->eq_in[1];
-          w[5] = ex->eq_in[5];
-          w[8] = ex->eq_in[8];
+          for (int i = 1; i < kQ; ++i)
+            if (bb & (1u << i)) w[i] = pq[kOpp[i]];
         }
-        if (prev_link & kLinkOutlet) {
-          w[3] = (prev_link & outlet_bb_bit(3)) ? s1[ring_index(kOpp[3]) * WP + tt]
-                                                : s1[ring_index(3) * WP + tt + 1];
-          w[6] = (prev_link & outlet_bb_bit(6)) ? s1[ring_index(kOpp[6]) * WP + tt]
-                                                : s2[ring_index(6) * WP + tt + 1];
-          w[7] = (prev_link & outlet_bb_bit(7)) ? s1[ring_index(kOpp[7]) * WP + tt]
-                                                : s0[ring_index(7) * WP + tt + 1];
+        if constexpr (kOpen) {
+          // apply_open_edges on the step-1 values: the inlet equilibrium, and at the outlet
+          // what x - 1 receives, R_{t+1}(i, x, q - c_y) (ring rows q, q - 1, q + 1 at column
+          // x) or R_{t+1}(opp(i), x - 1, q) (ring row q at column x - 1)
+          if (prev_link & kLinkInlet) {
+            w[1] = ex->eq_in[1];
+            w[5] = ex->eq_in[5];
+            w[8] = ex->eq_in[8];
+          }
+          if (prev_link & kLinkOutlet) {
+            w[3] = (prev_link & outlet_bb_bit(3)) ? s1[ring_index(kOpp[3]) * WP + tt]
+                                                  : s1[ring_index(3) * WP + tt + 1];
+            w[6] = (prev_link & outlet_bb_bit(6)) ? s1[ring_index(kOpp[6]) * WP + tt]
+                                                  : s2[ring_index(6) * WP + tt + 1];
+            w[7] = (prev_link & outlet_bb_bit(7)) ? s1[ring_index(kOpp[7]) * WP + tt]
+                                                  : s0[ring_index(7) * WP + tt + 1];
+          }
         }
+        if constexpr (!kAhead) vs = (prev_link & kLinkSolid) ? tail - so : vo;
       }
       typename P::Word out[kQ];
-      const int q = y0 - 2 + k;
       if constexpr (kField)
         cells.step2(w, f2x, f2y, out);
       else
         cells.step2(w, prev_carry, out);
-      const unsigned so = static_cast<unsigned>(
-          __builtin_amdgcn_readfirstlane(static_cast<unsigned>(q * nx) * B));
-      const unsigned vs = (prev_link & kLinkSolid) ? tail - so : vo;
+      if constexpr (kAhead) {
+        if (prev_flag) vs = (prev_link & kLinkSolid) ? tail - so : vo;
+      }
 #pragma unroll
       for (int i = 0; i < kQ; ++i) P::template store<kStoreAux>(gn, i, vs, so, out[i]);
       if constexpr (kHalo) {
@@ -598,6 +618,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
       cur[i] = nxt[i];
     }
     prev_link = link;
+    prev_flag = flag;
     prev_carry = carry;
     if constexpr (kField) {
       f2x = f1x;
