@@ -131,6 +131,9 @@ struct PlainCells {
   // before it, so they are not in flight across step 1's collision: 8192^2 fp64 70.8k -> 71.3k
   // MLUPS, 8192 x 6144 67.5k -> 70.1k; float and FP16 measured no different either way
   static constexpr bool kPullLate = sizeof(T) == 8;
+  // rows whose pulls are in flight ahead of the current one: double 2 (8192^2 +0.2-0.3 %, issued
+  // late: early measured the same), float 1 (2 measured 0.3-0.4 % slower, strip included)
+  static constexpr int kDepth = sizeof(T) == 8 ? 2 : 1;
   T omega, fp, fx, fy;
   __device__ __forceinline__ uint32_t step1(Ring (&v)[kQ], uint32_t) const {
     collide(v, fx, fy, omega, fp);
@@ -165,10 +168,13 @@ struct HalfCells {
   using S = uint16_t;
   using Ring = _Float16;
   using Word = unsigned short;
-  // the FP16 walk keeps the whole step-1 row in registers (88 VGPRs; reading it from the ring
-  // measured 109)
+  // the FP16 walk keeps the whole step-1 row in registers (86 VGPRs with one row of pulls in
+  // flight; reading it from the ring measured 109)
   static constexpr bool kRowInRegisters = true;
   static constexpr bool kPullLate = false;
+  // two rows of pulls in flight (binary16: 128 B per wave and load): 8192^2 +0.6-1.3 %, 95 VGPRs,
+  // still five waves per SIMD
+  static constexpr int kDepth = 2;
   HalfCollide col;
   uint32_t seed;  // seed of the step pair
   __device__ __forceinline__ uint32_t step1(Ring (&v)[kQ], uint32_t cell) const {
@@ -390,7 +396,12 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
   // is 0, so its step 2 needs no bounce-back, open-edge or solid-cell handling)
   uint32_t prev_flag = 1;
   uint32_t prev_carry = 0;  // what step 1 at (c, r - 1) handed to its step 2
+  // pulls in flight: of the next row (kDepth 1), or of the next two rows (kDepth 2: more bytes
+  // in flight per wave, at the cost of nine registers; the batched walk's bounce-back data and
+  // a force field's rows are loaded one row ahead, so those walks keep one)
+  constexpr int kDepth = (kAhead || kField) ? 1 : P::kDepth;
   T cur[kQ];            // raw pulls of the current step-1 row
+  T nx1[kQ];            // kDepth 2: raw pulls of the next row
   auto pull = [&](const Rows& rw, T (&out)[kQ]) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) out[i] = g.template load_v<S>(i, cols.of(kCx[i]), rw.of(kCy[i]));
@@ -421,6 +432,7 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     const int r0 = wrap_y(y0 - 1);
     const Rows rows0 = uniform_rows_of(r0, nx, ny, B);
     pull(rows0, cur);
+    if constexpr (kDepth == 2) pull(uniform_rows_of(wrap_y(y0), nx, ny, B), nx1);
     if constexpr (kAhead) bounce_data(r0, rows0);
     if constexpr (kField) {
       f1x = Stored<S>::load(fxr, cols.c, rows0.c);
@@ -428,22 +440,25 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
     }
   };
 
-  // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row k + 1
-  // are issued first (P::kPullLate: after step 1, before the barrier) and nothing between them
-  // and their use branches around a vector memory
-  // operation (the bounce-back loads complete inside their own branch, solid cells store to
-  // the discard slots): one wait counter covers loads and stores, and a branch around either
-  // would make the wait for the pulls a wait for every store in flight.
+  // Step-1 row k (ring slot k & 3) and, when `two`, step 2 at row k - 1. The pulls of row
+  // k + kDepth are issued first (P::kPullLate: after step 1, before the barrier) and nothing
+  // between them and their use branches around a vector memory operation (the bounce-back loads
+  // complete inside their own branch, solid cells store to the discard slots): one wait counter
+  // covers loads and stores, and a branch around either would make the wait for the pulls a
+  // wait for every store in flight.
   auto row = [&](const int k, const int hk, const int slot, const bool two)
                  __attribute__((always_inline)) {
     const int r = wrap_y(y0 - 1 + k);
     const Rows rows = uniform_rows_of(r, nx, ny, B);
-    T nxt[kQ];  // the last row re-reads its own rows (cached) instead of branching
+    T nxt[kQ];  // the last rows re-read their own rows (cached) instead of branching
     const int rn = k + 1 < nr ? wrap_y(y0 + k) : r;
     const Rows rows_n = uniform_rows_of(rn, nx, ny, B);
+    // the row whose pulls this row issues
+    const Rows rows_p = kDepth == 1 ? rows_n
+                                    : uniform_rows_of(k + 2 < nr ? wrap_y(y0 + k + 1) : r, nx, ny, B);
     S fnx = S(0), fny = S(0);
     auto issue_next = [&]() __attribute__((always_inline)) {
-      pull(rows_n, nxt);
+      pull(rows_p, nxt);
       if constexpr (kField) {
         fnx = Stored<S>::load(fxr, cols.c, rows_n.c);
         fny = Stored<S>::load(fyr, cols.c, rows_n.c);
@@ -615,7 +630,12 @@ __device__ __forceinline__ void step2_walk(const Step2View<typename P::S>& p, co
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       if (P::kRowInRegisters || kCx[i] == 0) prev[i] = v[i];
-      cur[i] = nxt[i];
+      if constexpr (kDepth == 1) {
+        cur[i] = nxt[i];
+      } else {
+        cur[i] = nx1[i];
+        nx1[i] = nxt[i];
+      }
     }
     prev_link = link;
     prev_flag = flag;
