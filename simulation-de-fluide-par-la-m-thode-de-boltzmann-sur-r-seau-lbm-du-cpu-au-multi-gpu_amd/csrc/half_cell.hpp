@@ -85,13 +85,15 @@ __device__ __forceinline__ void round_cell(const float (&v)[kQ], float mass, uin
 __device__ __forceinline__ constexpr int wclass(int i) { return i == 0 ? 0 : (i < 5 ? 1 : 2); }
 
 // BGK + Guo in scaled units with the constants of HalfCollide: per pair of opposite directions
-// one even and one odd part (9 operations for two directions). The polynomial 4.5 cu^2 - 1.5 u.u
-// is formed per direction, so its two terms cancel at their own small magnitude. Every fused
-// multiply-add is written out (no contraction left to the compiler), so the one-step and
-// two-step FP16 kernels perform the same operations on a cell; sums with an operand that may
-// come straight from binary16 storage are fma(a, unit, b). Returns the cell's mass S drho (the
-// sum of its nine pulled values). v[0] is left as it was: round_cell sets the rest population
-// from the mass.
+// one even and one odd part (7 operations for two directions). With b = omega w S rho the even
+// part b (4.5 cu^2 - 1.5 u.u) + B_i cu + omega w S drho - C u.F is evaluated as
+// (4.5 b cu + B_i) cu + e_k, its direction-independent terms e_k gathered once per weight class:
+// fp32 rounding differs from forming the polynomial first by far less than the binary16
+// storage resolves. Every fused multiply-add is written out (no contraction left to the
+// compiler), so the one-step and two-step FP16 kernels perform the same operations on a cell;
+// sums with an operand that may come straight from binary16 storage are fma(a, unit, b).
+// Returns the cell's mass S drho (the sum of its nine pulled values). v[0] is left as it was:
+// round_cell sets the rest population from the mass.
 __device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[kQ]) {
   const float one = c.unit;
   auto add = [one](float a, float b) { return __builtin_fmaf(a, one, b); };
@@ -106,15 +108,15 @@ __device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[k
   const float irS = __builtin_amdgcn_rcpf(rho) * c.invS;
   const float ux = (mx + c.hfxS) * irS;
   const float uy = (my + c.hfyS) * irS;
-  const float usq15 = 1.5f * __builtin_fmaf(ux, ux, uy * uy);
+  const float rusq = rho * __builtin_fmaf(ux, ux, uy * uy);
   const float uF = __builtin_fmaf(ux, c.fx, uy * c.fy);
   // per moving weight class (0 axis, 1 diagonal; the rest direction is not updated here):
-  // omega w S drho - C u.F, and omega w S rho and 3 omega w S rho
-  float ac[2], b[2], b3[2];
+  // e_k = omega w S (drho - 1.5 rho u.u) - C u.F, and 4.5 and 3 omega w S rho
+  float e[2], b45[2], b3[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    ac[k] = __builtin_fmaf(c.ow[k + 1], m0, -(c.C[k + 1] * uF));
-    b[k] = c.owS[k + 1] * rho;
+    e[k] = __builtin_fmaf(-c.owS15[k + 1], rusq, __builtin_fmaf(c.ow[k + 1], m0, -(c.C[k + 1] * uF)));
+    b45[k] = c.owS45[k + 1] * rho;
     b3[k] = c.owS3[k + 1] * rho;
   }
   // pairs (direction, opposite) with the direction's cu: (1, 3) ux, (2, 4) uy, (5, 7) ux + uy,
@@ -126,8 +128,7 @@ __device__ __forceinline__ float fast_collide(const HalfCollide& c, float (&v)[k
   for (int j = 0; j < 4; ++j) {
     const int i = kFirst[j], o = kOpp[i], k = wclass(i) - 1;
     const float cu = cus[j];
-    const float poly = __builtin_fmaf(4.5f * cu, cu, -usq15);
-    const float even = __builtin_fmaf(b[k], poly, __builtin_fmaf(c.B[i], cu, ac[k]));
+    const float even = __builtin_fmaf(__builtin_fmaf(b45[k], cu, c.B[i]), cu, e[k]);
     const float odd = __builtin_fmaf(b3[k], cu, c.A[i]);
     v[i] = __builtin_fmaf(c.om1, v[i], even + odd);
     v[o] = __builtin_fmaf(c.om1, v[o], even - odd);

@@ -58,7 +58,7 @@ struct StepParams {
 // (the compiler turns a literal fma(a, 1, b) back into an addition).
 struct HalfCollide {
   float om1, invS, hfxS, hfyS, fx, fy, unit;  // hf.S = half the force in scaled units
-  float ow[3], owS[3], owS3[3];  // omega w_k, omega w_k S, 3 omega w_k S
+  float ow[3], owS15[3], owS3[3], owS45[3];  // omega w_k and omega w_k S times 1.5, 3 and 4.5
   float A[9], B[9], C[3];
 
   __host__ __device__ void setup(float omega, float fp, float S, float inv_s, float fx_, float fy_) {
@@ -78,8 +78,9 @@ struct HalfCollide {
     for (int k = 0; k < 3; ++k) {
       const float w = weight<float>(k == 0 ? 0 : (k == 1 ? 1 : 5));
       ow[k] = omega * w;
-      owS[k] = omega * w * S;
+      owS15[k] = 1.5f * omega * w * S;
       owS3[k] = 3.0f * omega * w * S;
+      owS45[k] = 4.5f * omega * w * S;
       C[k] = 3.0f * S * fp * w;
     }
   }
