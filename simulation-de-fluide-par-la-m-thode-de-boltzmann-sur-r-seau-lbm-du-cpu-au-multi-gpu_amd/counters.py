@@ -25,6 +25,16 @@ Run on a machine with the GPU, from the repository root (the program itself righ
         -- python3 bench.py --steps 6 --warmup 2 --no-cpu --forms copy_ceiling
 
 or, on CSV files that rocprofv3 wrote elsewhere, `summarize --fetch F.csv --write W.csv ...`.
+
+`waves` reads the wave-state counters instead (two passes of eight SQ counters, WAVE_PASSES) and
+reports per kernel the engine clock under the load (SQ_BUSY_CYCLES over the shader engines and
+the dispatch time), the fraction of the SIMD time the vector ALUs were active
+(SQ_ACTIVE_INST_VALU quad-cycles over SIMDs x time x clock / 4) and the resident waves per SIMD
+(SQ_WAVE_CYCLES over the same quad-cycles):
+
+    python <package>/counters.py waves --kernel step2_fp16_kernel --kernel 'step2_kernel<double' \
+        --last 90 --out profiles/step2_8192x8192.waves.md \
+        -- python3 bench.py --steps 20 --warmup 4 --settle 200 --no-cpu --forms precisions
 """
 from __future__ import annotations
 
@@ -43,6 +53,15 @@ import time
 HERE = pathlib.Path(__file__).resolve().parent
 CALIBRATION_KERNEL = "copy_kernel<4, true>"  # 16-byte non-temporal grid-stride copy
 CALIBRATION_BYTES = 2_415_919_104            # bytes read (and written) per copy dispatch
+# Wave-state passes (at most 8 SQ counters per rocprofv3 pass) and the MI355X shape they are
+# normalised by: 256 CUs of 4 SIMDs, 32 shader engines (8 XCDs of 4), 2.4 GHz peak engine clock.
+WAVE_PASSES = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY "
+               "SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_INSTS_SALU",
+               "SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_ACTIVE_INST_ANY "
+               "SQ_INST_CYCLES_VMEM_RD SQ_WAIT_INST_LDS SQ_ACTIVE_INST_MISC")
+SIMDS = 1024
+SHADER_ENGINES = 32
+PEAK_CLOCK_HZ = 2.4e9
 
 
 def source_tag() -> str:
@@ -75,14 +94,20 @@ def read_counters(path: str | os.PathLike) -> list[dict]:
     ci = _col(head, "Counter_Name", "CounterName")
     vi = _col(head, "Counter_Value", "CounterValue", "Value")
     di = _col(head, "Dispatch_Id", "DispatchId", "Correlation_Id", "Index")
+    try:  # dispatch start and end (ns), where the file has them
+        ti = (_col(head, "Start_Timestamp"), _col(head, "End_Timestamp"))
+    except ValueError:
+        ti = None
     acc: dict[tuple, dict] = {}
     for r in rows[1:]:
-        if len(r) <= max(ki, ci, vi, di):
+        if len(r) <= max(ki, ci, vi, di, *(ti or ())):
             continue
         key = (r[di], r[ci])
         e = acc.setdefault(key, {"dispatch": int(float(r[di])), "kernel": r[ki], "counter": r[ci],
                                  "value": 0.0})
         e["value"] += float(r[vi])
+        if ti:
+            e["ns"] = float(r[ti[1]]) - float(r[ti[0]])
     return sorted(acc.values(), key=lambda e: e["dispatch"])
 
 
@@ -136,6 +161,73 @@ def summarize(fetch_rows, write_rows, kernel: str, cells: int, updates: int,
             out["valu_instructions_per_launch"] = round(_mean(v) * k)
             out["valu_instructions_per_update"] = round(_mean(v) * k * 64 / (cells * updates), 1)
     return out
+
+
+def wave_states(rows: list[dict], kernel: str, skip: int = 0, last: int = 0,
+                simds: int = SIMDS, shader_engines: int = SHADER_ENGINES) -> dict:
+    """Wave-state figures of `kernel` from the rows of the SQ passes (the counters of both
+    passes read into one list): per dispatch means of each counter over the selected dispatches,
+    the mean dispatch time of each pass, the engine clock SQ_BUSY_CYCLES / shader engines / time,
+    and, at that clock, the SIMD time in quad-cycles (simds x time x clock / 4) over which
+    SQ_ACTIVE_INST_VALU gives the vector ALUs' busy fraction and SQ_WAVE_CYCLES the resident waves
+    per SIMD."""
+    names = sorted({e["counter"] for e in rows if kernel in e["kernel"]})
+    if not names:
+        raise ValueError("no dispatch of the kernel in the counter file")
+    mean = {c: _mean(per_dispatch(rows, kernel, c, skip, last)) for c in names}
+
+    def seconds(counter):
+        sel = [e for e in rows if kernel in e["kernel"] and e["counter"] == counter][skip:]
+        sel = sel[-last:] if last > 0 else sel
+        ns = [e["ns"] for e in sel if "ns" in e]
+        return _mean(ns) * 1e-9 if ns else None
+
+    out = {"kernel": kernel, "counters": {c: mean[c] for c in names},
+           "dispatches": len(per_dispatch(rows, kernel, names[0], skip, last))}
+    t = seconds("SQ_BUSY_CYCLES")
+    if t and "SQ_BUSY_CYCLES" in mean:
+        clock = mean["SQ_BUSY_CYCLES"] / shader_engines / t
+        quads = simds * t * clock / 4
+        out.update(dispatch_seconds=t, clock_hz=clock, simd_quad_cycles=quads)
+        if "SQ_ACTIVE_INST_VALU" in mean:
+            out["valu_active_fraction"] = mean["SQ_ACTIVE_INST_VALU"] / quads
+            out["valu_active_fraction_at_peak_clock"] = (
+                mean["SQ_ACTIVE_INST_VALU"] / (simds * t * PEAK_CLOCK_HZ / 4))
+        if "SQ_WAVE_CYCLES" in mean:
+            out["waves_per_simd"] = mean["SQ_WAVE_CYCLES"] / quads
+        if "SQ_INSTS_VALU" in mean:
+            out["valu_instruction_fraction"] = mean["SQ_INSTS_VALU"] / quads
+    return out
+
+
+def write_waves(path: str | os.PathLike, results: list[dict], commands: list[str]) -> None:
+    md = ["# Wave states (rocprofv3 SQ counters)", "",
+          f"Written by `counters.py` on {time.strftime('%Y-%m-%d %H:%M')} UTC; device sources "
+          f"tag `{source_tag()}`. Per dispatch, means over {results[0]['dispatches']} dispatches "
+          f"of each kernel; {SIMDS} SIMDs, {SHADER_ENGINES} shader engines. Commands (repository "
+          "root):", ""]
+    md += [f"    {c}" for c in commands]
+    md += ["", "| quantity | " + " | ".join(f"`{r['kernel']}`" for r in results) + " |",
+           "|---|" + "---|" * len(results)]
+
+    def row(label, f):
+        md.append(f"| {label} | " + " | ".join(f(r) for r in results) + " |")
+
+    def opt(key, fmt):
+        return lambda r: fmt.format(r[key]) if key in r else "-"
+
+    row("dispatch time (ms)", lambda r: f"{r['dispatch_seconds'] * 1e3:.4f}" if "dispatch_seconds" in r else "-")
+    row("engine clock (GHz, SQ_BUSY_CYCLES)", lambda r: f"{r['clock_hz'] / 1e9:.3f}" if "clock_hz" in r else "-")
+    row("vector ALUs active (at that clock; at 2.4 GHz)",
+        lambda r: (f"{r['valu_active_fraction']:.3f}; {r['valu_active_fraction_at_peak_clock']:.3f}"
+                   if "valu_active_fraction" in r else "-"))
+    row("VALU instructions / SIMD quad-cycles", opt("valu_instruction_fraction", "{:.3f}"))
+    row("resident waves per SIMD", opt("waves_per_simd", "{:.2f}"))
+    for c in sorted({c for r in results for c in r["counters"]}):
+        row(c, lambda r, c=c: f"{r['counters'][c]:.4e}" if c in r["counters"] else "-")
+    md += ["", "```json", json.dumps({"source_tag": source_tag(), "kernels": results}, indent=1),
+           "```", ""]
+    pathlib.Path(path).write_text("\n".join(md))
 
 
 def write_summary(path: str | os.PathLike, s: dict, commands: list[str]) -> None:
@@ -216,10 +308,39 @@ def main(argv=None) -> None:
     s.add_argument("--fetch", required=True)
     s.add_argument("--write", required=True)
     s.add_argument("--valu-csv", default=None)
+    wv = sub.add_parser("waves", help="wave-state counters (two SQ passes) per kernel")
+    wv.add_argument("--kernel", action="append", required=True,
+                    help="substring of a kernel name (repeat for several kernels)")
+    wv.add_argument("--skip", type=int, default=0, help="dispatches left out at the start")
+    wv.add_argument("--last", type=int, default=0, help="only the last N dispatches (after --skip)")
+    wv.add_argument("--out", required=True, help="summary Markdown file")
+    wv.add_argument("--work", default="rocprof_counters", help="rocprofv3 output directory")
+    wv.add_argument("--timeout", type=int, default=300)
+    wv.add_argument("--csv", action="append", default=None,
+                    help="summarise these counter CSV files instead of running the passes")
+    wv.add_argument("cmd", nargs=argparse.REMAINDER, help="-- program and arguments")
     a = ap.parse_args(argv)
     rel = pathlib.Path(__file__).resolve().relative_to(pathlib.Path.cwd()) \
         if pathlib.Path.cwd() in pathlib.Path(__file__).resolve().parents else pathlib.Path(__file__)
     made_by = " ".join(shlex.quote(c) for c in ["python3", str(rel), *(argv or sys.argv[1:])])
+    if a.what == "waves":
+        rows, commands = [], [made_by]
+        if a.csv:
+            for c in a.csv:
+                rows += read_counters(c)
+        else:
+            cmd = a.cmd[1:] if a.cmd and a.cmd[0] == "--" else a.cmd
+            if not cmd:
+                ap.error("give the program after --, or --csv files")
+            commands.append("  which runs:")
+            for k, counters in enumerate(WAVE_PASSES):
+                csv_path, line = run_pass(counters, f"{a.work}/sq{k}", cmd, a.timeout)
+                commands.append(line)
+                rows += read_counters(csv_path)
+        results = [wave_states(rows, k, a.skip, a.last) for k in a.kernel]
+        write_waves(a.out, results, commands)
+        print(json.dumps(results), flush=True)
+        return
     if a.what == "run":
         cmd = a.cmd[1:] if a.cmd and a.cmd[0] == "--" else a.cmd
         if not cmd:

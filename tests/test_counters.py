@@ -102,3 +102,38 @@ def test_launch_of_several_dispatches(tmp_path, counters):
         counters.summarize(counters.read_counters(tmp_path / "f.csv"),
                            counters.read_counters(tmp_path / "w.csv"), "step2_kernel", cells, 2,
                            144, last=10, dispatches_per_launch=3)
+
+
+def test_wave_states(tmp_path, counters):
+    """Clock, vector-ALU busy fraction and resident waves from the two SQ passes: a dispatch of
+    1 ms whose 32 shader engines count 2.2e9 busy cycles each per second runs at 2.2 GHz; the
+    1,024 SIMDs then offer 1024 x 1e-3 x 2.2e9 / 4 quad-cycles."""
+    def write(path, rows, ns):
+        with open(path, "w", newline="") as f:
+            w = csv.writer(f)
+            w.writerow(HEAD)
+            for d, (k, c, v) in enumerate(rows):
+                r = ["0"] * len(HEAD)
+                r[1], r[8], r[15], r[16] = str(d // 2 + 1), k, c, repr(v)
+                r[17], r[18] = "1000", str(1000 + ns)
+                w.writerow(r)
+    t, clock = 1e-3, 2.2e9
+    quads = 1024 * t * clock / 4
+    p0 = [(STEP, "SQ_BUSY_CYCLES", 32 * clock * t), (STEP, "SQ_ACTIVE_INST_VALU", 0.9 * quads)] * 3
+    p1 = [(STEP, "SQ_INSTS_VALU", 0.8 * quads), (STEP, "SQ_WAVE_CYCLES", 3.5 * quads)] * 3
+    write(tmp_path / "a.csv", p0, int(t * 1e9))
+    write(tmp_path / "b.csv", p1, int(t * 1e9))
+    rows = counters.read_counters(tmp_path / "a.csv") + counters.read_counters(tmp_path / "b.csv")
+    r = counters.wave_states(rows, "step2_kernel<float", last=2)
+    assert r["dispatches"] == 2
+    assert r["clock_hz"] == pytest.approx(clock)
+    assert r["valu_active_fraction"] == pytest.approx(0.9)
+    assert r["valu_active_fraction_at_peak_clock"] == pytest.approx(0.9 * 2.2 / 2.4)
+    assert r["valu_instruction_fraction"] == pytest.approx(0.8)
+    assert r["waves_per_simd"] == pytest.approx(3.5)
+    out = tmp_path / "w.md"
+    counters.write_waves(out, [r], ["rocprofv3 --pmc SQ_WAVES ..."])
+    text = out.read_text()
+    assert "| engine clock (GHz, SQ_BUSY_CYCLES) | 2.200 |" in text
+    with pytest.raises(ValueError, match="no dispatch"):
+        counters.wave_states(rows, "nothing_like_this")
